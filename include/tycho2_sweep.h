@@ -1,0 +1,162 @@
+/*
+ * tycho2_sweep.h -- C ABI of the MI355X sweep engine (libtycho2_sweep.so).
+ *
+ * The engine replaces, for one call of SweeperAbstract::sweep(psi, source)
+ * and for the outer iterations that drive it, the reference's sweep stack:
+ *   SweeperAbstract / SweeperTraverse  src/SweeperAbstract.hh:47-70,
+ *                                      src/SweeperTraverse.cc:55-89
+ *   GraphTraverser + Priorities        src/GraphTraverser.cc:416-759,
+ *                                      src/Priorities.cc:396-435
+ *   SweepData + Transport cell solve   src/SweepData.hh:139-174,
+ *                                      src/Transport.cc:687-776
+ *   PsiData / PhiData storage          src/PsiData.hh:59-371
+ *   moments + outer iterations         src/Util.cc:85-159,
+ *                                      src/SourceIteration.cc:76-285
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Host arrays use the reference's layouts:
+ *       psi, source  [cell][angle][vertex][group]  (src/PsiData.hh:162)
+ *       phi          [cell][vertex][group]         (src/PsiData.hh:369)
+ *       sigma        [cell][group]
+ *     Mesh indices are int64 with -1 where the reference uses UINT64_MAX.
+ *   - Every entry point returns T2_OK (0) or a negative status; the message
+ *     of the last failure on the calling thread is t2_last_error().  No C++
+ *     exception crosses the boundary (the reference aborts instead,
+ *     src/Assert.cc:51-65).
+ *   - One host thread at a time per handle (the reference is
+ *     MPI_THREAD_SINGLE, src/Main.cc:197-201).
+ */
+#ifndef TYCHO2_SWEEP_H
+#define TYCHO2_SWEEP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define T2_ABI_VERSION 1
+
+/* status codes */
+#define T2_OK 0
+#define T2_ERR_ARG -1      /* bad argument or inconsistent mesh            */
+#define T2_ERR_HIP -2      /* HIP runtime failure                          */
+#define T2_ERR_NOMEM -3    /* device or host allocation failed             */
+#define T2_ERR_STATE -4    /* call not valid in the handle's current state */
+#define T2_ERR_NODEV -5    /* no usable GPU                                */
+#define T2_ERR_NOCONV -6   /* iteration hit its limit without converging   */
+
+/* sweep arithmetic (also the angular-flux storage type) */
+#define T2_FP64 0
+#define T2_FP32 1
+
+/* cell solvers, deck key GaussElim (src/Main.cc:167-176) */
+#define T2_GAUSS_NOPIVOT 1
+#define T2_GAUSS_ORIGINAL 0
+
+typedef struct t2_sweeper t2_sweeper;
+
+typedef struct {
+    int64_t n_cells, n_angles, n_groups;
+    int64_t n_steps;          /* sweep steps (max levels over angles)      */
+    int64_t min_levels;       /* fewest levels of any angle                */
+    int64_t n_lagged;         /* (angle, cell, face) triples lagged        */
+    int64_t psi_bytes;        /* device bytes of the angular flux          */
+    int64_t device_bytes;     /* all device memory held by the handle      */
+    int32_t precision;        /* T2_FP64 / T2_FP32                         */
+    int32_t gauss;            /* T2_GAUSS_*                                */
+} t2_info;
+
+/* Message of the last failure on this thread ("" if none). */
+const char *t2_last_error(void);
+int t2_abi_version(void);
+
+/* ------------------------------------------------------------ host helpers
+ * Pure host functions (no device work); usable without a GPU. */
+
+/* Cell volumes, face areas and outward unit normals as the reference's
+ * TychoMesh constructor computes them (src/TychoMesh.cc:52-208), with the
+ * face vertex order f2c[c][k][j] (src/TychoMesh.hh:99-100). */
+int t2_mesh_geometry(int64_t n_cells, int64_t n_nodes, const double *coords,
+                     const int64_t *cell_nodes, const int64_t *f2c,
+                     double *volume, double *area, double *normal);
+
+/* Level of every (angle, cell) in the per-angle sweep order, levels[a][c]
+ * (replaces GraphTraverser's ready queue, src/GraphTraverser.cc:467-644).
+ * Returns the number of lagged faces in *n_lagged (cycles, see DESIGN). */
+int t2_sweep_levels(int64_t n_cells, const int64_t *adj_cell,
+                    const double *normal, int64_t n_angles,
+                    const double *omega, int32_t *levels, int64_t *n_lagged);
+
+/* ------------------------------------------------------------ setup
+ * Replaces the reference's per-mesh set-up (TychoMesh, Quadrature,
+ * GraphTraverser constructor, Priorities::calcPriorities; src/Main.cc:
+ * 245-281).  Mesh: node coordinates, cell nodes, cell adjacency (-1 on the
+ * boundary) and face-to-cell vertex map as TychoMesh exposes them
+ * (getNodeCoord, getCellNode, getAdjCell, getFaceToCellVrtx).  Quadrature:
+ * omega[a][3] and weights[a].  Cross sections sigma[cell][group]. */
+int t2_sweeper_create(t2_sweeper **out, int device,
+                      int64_t n_cells, int64_t n_nodes, const double *coords,
+                      const int64_t *cell_nodes, const int64_t *adj_cell,
+                      const int64_t *f2c, int64_t n_angles,
+                      const double *omega, const double *weights,
+                      int64_t n_groups, const double *sigma_t,
+                      const double *sigma_s, int precision, int gauss);
+int t2_sweeper_destroy(t2_sweeper *s);
+int t2_sweeper_info(const t2_sweeper *s, t2_info *info);
+
+/* Launch shape: cells per block and groups per lane (autotuner knobs);
+ * 0 keeps the default. */
+int t2_set_launch(t2_sweeper *s, int items_per_block, int groups_per_lane);
+
+/* ------------------------------------------------------------ drop-in sweep
+ * SweeperAbstract::sweep(psi, source, zeroPsiBound) on host arrays in the
+ * reference layout: psi = L^-1 source for all angles and groups.  psi is
+ * written (its old contents are read only through lagged faces).
+ * zero_psi_bound has no effect on one device (no partition faces). */
+int t2_sweep_host(t2_sweeper *s, double *psi, const double *source,
+                  int zero_psi_bound);
+
+/* ------------------------------------------------------------ device loop
+ * The fixed source of SweeperAbstract::solve(): isotropic q[cell][vertex]
+ * [group], or the reference's angle-dependent source[cell][angle][vertex]
+ * [group] (host arrays; kept on the device).  Setting one clears the other;
+ * NULL clears both. */
+int t2_set_source_isotropic(t2_sweeper *s, const double *q);
+int t2_set_source_angular(t2_sweeper *s, const double *source);
+
+/* Source iteration on the device (src/SourceIteration.cc:127-207): starts
+ * from psi = 0, stops when the L1-relative phi change <= err_max or after
+ * iter_max sweeps.  err_hist (host, length iter_max) may be NULL. */
+int t2_solve_source_iteration(t2_sweeper *s, int64_t iter_max, double err_max,
+                              int64_t *n_iter, double *err_hist);
+
+/* GMRES(restart) on (I - D L^-1 M S) phi = D L^-1 q with x0 = 0, then one
+ * sweep for psi (src/SourceIteration.cc:216-285).  Arnoldi, Gram-Schmidt and
+ * the iterate are FP64 on the device whatever the sweep precision.
+ * res_hist (host, length iter_max) may be NULL. */
+int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,
+                    int64_t restart, int64_t *n_iter, double *res_hist);
+
+/* One sweep with the current scalar flux: psi = L^-1 (q + S phi) and
+ * phi <- D psi (one source-iteration step; the benchmark's unit of work). */
+int t2_sweep_step(t2_sweeper *s);
+
+/* Copy results to the host in the reference layouts. */
+int t2_get_phi(const t2_sweeper *s, double *phi);
+int t2_get_psi(const t2_sweeper *s, double *psi);
+/* One angle of psi, psi_a[cell][vertex][group] (e.g. to inspect full-size
+ * problems whose whole psi does not fit the host). */
+int t2_get_psi_angle(const t2_sweeper *s, int64_t angle, double *psi_a);
+/* Replace the device scalar flux (host phi[cell][vertex][group]). */
+int t2_set_phi(t2_sweeper *s, const double *phi);
+
+/* Synchronize the handle's stream; returns the HIP stream it launches on
+ * in *stream (a hipStream_t) for event timing by the caller. */
+int t2_synchronize(t2_sweeper *s);
+int t2_get_stream(const t2_sweeper *s, void **stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,168 @@
+"""Parity of the HIP sweep engine against the CPU oracle (FP64 bar 1e-12).
+
+All calls go through the C ABI (tycho2_amd/libtycho2_sweep.so); the oracle
+only checks.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from tycho2_amd import mesh as meshmod
+from tycho2_amd import problems
+from tycho2_amd.psi_io import rel_diff
+from tycho2_amd.quadrature import product_quadrature
+from tycho2_amd.sweeper import FP32, FP64, SweeperHIP
+
+pytestmark = pytest.mark.gpu
+
+
+def maxrel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def random_problem(n=3, sn=4, ng=4, seed=0, jitter=0.15):
+    sm = meshmod.kuhn_cube(n, jitter, seed=seed)
+    conn = meshmod.tycho_connectivity(sm)
+    omega, wt = product_quadrature(sn)
+    rng = np.random.default_rng(seed)
+    st = rng.uniform(0.5, 3.0, (sm.n_cells, ng))
+    ss = st * rng.uniform(0.1, 0.9, (sm.n_cells, ng))
+    return sm, conn, omega, wt, st, ss, rng
+
+
+@pytest.mark.parametrize("ng", [1, 2, 4, 8, 16, 32])
+def test_single_sweep_fp64(ng):
+    print(f"single sweep fp64 ng={ng}", flush=True)
+    sm, conn, omega, wt, st, ss, rng = random_problem(ng=ng, seed=ng)
+    src = rng.uniform(-0.5, 2.0, (sm.n_cells, omega.shape[0], 4, ng))
+    psi = np.zeros_like(src)
+    with SweeperHIP(sm, conn, omega, wt, ng, st, ss, FP64) as s:
+        s.sweep(psi, src)
+        assert s.info()["n_lagged"] == 0
+    ref, lag = orc.sweep(orc.build_mesh(sm), omega, ng, st, src)
+    assert lag == 0
+    assert maxrel(psi, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("ng", [1, 16])
+def test_single_sweep_fp32(ng):
+    print(f"single sweep fp32 ng={ng}", flush=True)
+    sm, conn, omega, wt, st, ss, rng = random_problem(ng=ng, seed=10 + ng)
+    src = rng.uniform(0.1, 2.0, (sm.n_cells, omega.shape[0], 4, ng))
+    psi = np.zeros_like(src)
+    with SweeperHIP(sm, conn, omega, wt, ng, st, ss, FP32) as s:
+        s.sweep(psi, src)
+    ref, _ = orc.sweep(orc.build_mesh(sm), omega, ng, st, src)
+    assert maxrel(psi, ref) <= 2e-5
+
+
+def test_launch_shapes_bitwise():
+    print("launch shapes", flush=True)
+    ng = 16
+    sm, conn, omega, wt, st, ss, rng = random_problem(n=4, ng=ng, seed=3)
+    src = rng.uniform(0.0, 1.0, (sm.n_cells, omega.shape[0], 4, ng))
+    outs = []
+    with SweeperHIP(sm, conn, omega, wt, ng, st, ss, FP64) as s:
+        for ipb, gpl in [(0, 1), (4, 1), (8, 1), (16, 2), (32, 4), (64, 4)]:
+            s.set_launch(ipb, gpl)
+            psi = np.zeros_like(src)
+            s.sweep(psi, src)
+            outs.append(psi)
+        with pytest.raises(Exception):
+            s.set_launch(64, 1)          # 1024 threads: above the bound
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
+def test_gold_source_iteration(cube208, gold):
+    """The reference's regression deck end to end on the device."""
+    print("gold source iteration", flush=True)
+    p = problems.gold(cube208)
+    om = orc.build_mesh(cube208)
+    src = orc.hat_source(om, p.omega, p.ng, 10.0, 5.0)
+    with SweeperHIP.from_problem(p) as s:
+        s.set_source_angular(src)
+        hist = s.solve_source_iteration(100, 1e-10)
+        psi = s.psi()
+    rpsi, _, rhist = orc.source_iteration(om, p.omega, p.wt, p.ng, 10.0, 5.0,
+                                          src, 100, 1e-10)
+    assert len(hist) == len(rhist)
+    assert maxrel(psi, rpsi) <= 1e-12
+    # against the reference's own answer, with its test/diff.py metric
+    got = psi[..., 0][:, gold["psi_angles"], :]
+    l1, linf = rel_diff(got, gold["psi_g0"])
+    assert l1 < 1e-10 and linf < 1e-10
+    assert np.array_equal(psi[..., 0], psi[..., 1])
+    assert abs(np.abs(psi).sum() / gold["psi_l1"] - 1) < 1e-10
+
+
+def test_plumbing_source_iteration():
+    """750 tets, 32 angles, 1 group, SI to 1e-12."""
+    print("plumbing SI", flush=True)
+    p = problems.plumbing()
+    om = orc.build_mesh(p.smesh)
+    src = orc.isotropic_source(p.nc, p.na, p.ng, np.ones((p.nc, p.ng)))
+    with SweeperHIP.from_problem(p) as s:
+        hist = s.solve_source_iteration(1000, 1e-12)
+        phi = s.phi()
+    _, rphi, rhist = orc.source_iteration(om, p.omega, p.wt, p.ng, 1.0, 0.5,
+                                          src, 1000, 1e-12)
+    assert len(hist) == len(rhist)
+    assert maxrel(phi, rphi) <= 1e-12
+
+
+def test_krylov_matches_source_iteration():
+    print("krylov vs SI", flush=True)
+    ng = 4
+    sm, conn, omega, wt, st, ss, rng = random_problem(n=4, ng=ng, seed=5)
+    q = np.ones((sm.n_cells, 4, ng))
+    src = orc.isotropic_source(sm.n_cells, omega.shape[0], ng,
+                               np.ones((sm.n_cells, ng)))
+    _, rphi, _ = orc.source_iteration(orc.build_mesh(sm), omega, wt, ng, st,
+                                      ss, src, 2000, 1e-14)
+    for prec, tol in ((FP64, 1e-10), (FP32, 1e-5)):
+        with SweeperHIP(sm, conn, omega, wt, ng, st, ss, prec) as s:
+            s.set_source_isotropic(q)
+            hist = s.solve_krylov(200, 1e-12, 20)
+            phi = s.phi()
+        assert len(hist) < 200
+        assert maxrel(phi, rphi) <= tol
+
+
+def test_sweep_step_matches_oracle():
+    print("sweep step", flush=True)
+    ng = 8
+    sm, conn, omega, wt, st, ss, rng = random_problem(n=4, ng=ng, seed=9)
+    q = rng.uniform(0.0, 1.0, (sm.n_cells, 4, ng))
+    phi0 = rng.uniform(0.0, 1.0, (sm.n_cells, 4, ng))
+    with SweeperHIP(sm, conn, omega, wt, ng, st, ss, FP64) as s:
+        s.set_source_isotropic(q)
+        s.set_phi(phi0)
+        s.sweep_step()
+        phi = s.phi()
+    src = np.ascontiguousarray(np.broadcast_to(
+        q[:, None], (sm.n_cells, omega.shape[0], 4, ng)))
+    m = orc.build_mesh(sm)
+    tot = src + (ss / (4 * np.pi))[:, None, None, :] * phi0[:, None]
+    psi, _ = orc.sweep(m, omega, ng, st, tot)
+    assert maxrel(phi, orc.psi_to_phi(psi, wt)) <= 1e-12
+
+
+def test_full_size_plain_angle_planes():
+    """384,000-tet, 128-angle, 16-group sweep; the oracle checks four angle
+    planes (sweeps of different angles are independent)."""
+    print("full-size plain sweep", flush=True)
+    p = problems.plain()
+    rng = np.random.default_rng(1)
+    phi0 = rng.uniform(0.0, 1.0, (p.nc, 4, p.ng))
+    angles = (0, 37, 90, 127)
+    with SweeperHIP.from_problem(p) as s:
+        s.set_phi(phi0)
+        s.sweep_step()
+        got = [s.psi_angle(a) for a in angles]
+    m = orc.build_mesh(p.smesh)
+    tot = (p.q_iso + (p.sigma_s / (4 * np.pi))[:, None, :] * phi0)
+    for a, psi_a in zip(angles, got):
+        ref, _ = orc.sweep(m, p.omega[a:a + 1], p.ng, p.sigma_t,
+                           tot[:, None])
+        assert maxrel(psi_a, ref[:, 0]) <= 1e-12

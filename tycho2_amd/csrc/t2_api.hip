@@ -1,0 +1,809 @@
+// C ABI of the sweep engine: device state, sweeps and outer iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "t2_common.hpp"
+#include "t2_kernels.hpp"
+
+namespace {
+
+using t2::fail;
+
+#define T2_HIP(expr)                                                        \
+    do {                                                                    \
+        hipError_t e_ = (expr);                                             \
+        if (e_ != hipSuccess)                                               \
+            return fail(e_ == hipErrorOutOfMemory ? T2_ERR_NOMEM : T2_ERR_HIP, \
+                        std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define T2_TRY(expr)        \
+    do {                    \
+        int rc_ = (expr);   \
+        if (rc_ != T2_OK)   \
+            return rc_;     \
+    } while (0)
+
+}  // namespace
+
+struct t2_sweeper {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t nc = 0, na = 0, ng = 0;
+    int precision = T2_FP64, gauss = T2_GAUSS_NOPIVOT;
+    int block = 256, gpl = 1;
+    bool use_graph = true;
+    t2::Schedule sched;
+    size_t bytes = 0;
+    std::vector<void *> allocs;
+
+    double *d_omega = nullptr, *d_wt = nullptr, *d_geo = nullptr;
+    double *d_sigs = nullptr;
+    void *d_sigt = nullptr;          // T [nc][G]
+    int4 *d_adj = nullptr;
+    uint32_t *d_nbr = nullptr, *d_work = nullptr;
+    void *d_psi = nullptr;           // T [na][nc][4][G]
+    void *d_src = nullptr;           // T [nc][4][G]
+    double *d_qiso = nullptr;        // FP64 [nc][4][G]
+    double *d_qang = nullptr;        // FP64 [na][nc][4][G]
+    double *d_stage = nullptr;       // FP64 psi-sized staging (host path)
+    double *d_hostang = nullptr;     // FP64 [na][nc][4][G] (host path)
+    double *d_phi = nullptr, *d_phi_old = nullptr;
+    double *d_red = nullptr, *d_scal = nullptr;
+    int64_t *d_lag_key = nullptr;
+    int32_t *d_lag_up = nullptr;
+    void *d_lag_buf = nullptr;
+    double *d_kry = nullptr;         // Krylov basis + b
+    int64_t kry_cols = 0;
+
+    // captured sweeps: [0] isotropic source only, [1] with an angular part
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    const double *graph_ang[2] = {nullptr, nullptr};
+
+    int64_t n_phi() const { return nc * 4 * ng; }
+    int64_t n_psi() const { return na * n_phi(); }
+    size_t tsize() const { return precision == T2_FP64 ? 8 : 4; }
+};
+
+namespace {
+
+int dalloc(t2_sweeper *s, void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0)
+        bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(T2_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) +
+                                      " B): " + hipGetErrorString(e));
+    }
+    s->allocs.push_back(*p);
+    s->bytes += bytes;
+    return T2_OK;
+}
+
+template <typename P>
+int dalloc_t(t2_sweeper *s, P **p, size_t bytes)
+{
+    void *v = nullptr;
+    int rc = dalloc(s, &v, bytes);
+    *p = static_cast<P *>(v);
+    return rc;
+}
+
+void dfree(t2_sweeper *s, void *p, size_t bytes)
+{
+    if (!p)
+        return;
+    auto it = std::find(s->allocs.begin(), s->allocs.end(), p);
+    if (it != s->allocs.end()) {
+        s->allocs.erase(it);
+        s->bytes -= bytes;
+    }
+    (void)hipFree(p);
+}
+
+int set_device(const t2_sweeper *s)
+{
+    T2_HIP(hipSetDevice(s->device));
+    return T2_OK;
+}
+
+void drop_graphs(t2_sweeper *s)
+{
+    for (int i = 0; i < 2; i++) {
+        if (s->graph[i])
+            (void)hipGraphExecDestroy(s->graph[i]);
+        s->graph[i] = nullptr;
+        s->graph_ang[i] = nullptr;
+    }
+}
+
+template <typename T>
+t2::SweepArgs<T> base_args(const t2_sweeper *s, const double *ang)
+{
+    t2::SweepArgs<T> a{};
+    a.cell_bits = s->sched.cell_bits;
+    a.nc = s->nc;
+    a.omega = s->d_omega;
+    a.geo = s->d_geo;
+    a.adj = s->d_adj;
+    a.nbr = s->d_nbr;
+    a.sigma_t = static_cast<const T *>(s->d_sigt);
+    a.src_iso = static_cast<const T *>(s->d_src);
+    a.src_ang = ang;
+    a.psi = static_cast<T *>(s->d_psi);
+    a.lag_key = s->d_lag_key;
+    a.n_lag = int64_t(s->sched.lagged.size());
+    a.lag_buf = static_cast<const T *>(s->d_lag_buf);
+    return a;
+}
+
+// Enqueue one whole sweep (lag snapshot + every step) on the stream.
+template <typename T>
+int enqueue_sweep(t2_sweeper *s, const double *ang)
+{
+    const int64_t n_lag = int64_t(s->sched.lagged.size());
+    if (n_lag > 0)
+        T2_HIP(t2::launch_lag_gather<T>(
+            s->d_lag_key, s->d_lag_up, n_lag, s->nc, int(s->ng), s->d_nbr,
+            static_cast<const T *>(s->d_psi), static_cast<T *>(s->d_lag_buf),
+            s->stream));
+    t2::SweepArgs<T> a = base_args<T>(s, ang);
+    for (int64_t st = 0; st < s->sched.n_steps; st++) {
+        const int64_t off = s->sched.step_off[st];
+        a.work = s->d_work + off;
+        a.n_items = s->sched.step_off[st + 1] - off;
+        T2_HIP(t2::launch_sweep_step<T>(a, int(s->ng), s->gpl, s->block,
+                                        s->stream));
+    }
+    return T2_OK;
+}
+
+int enqueue_sweep_any(t2_sweeper *s, const double *ang)
+{
+    return s->precision == T2_FP64 ? enqueue_sweep<double>(s, ang)
+                                   : enqueue_sweep<float>(s, ang);
+}
+
+// psi = L^-1 (src_iso + ang); captured once into a hipGraph and replayed.
+int run_sweep(t2_sweeper *s, const double *ang)
+{
+    if (!s->use_graph)
+        return enqueue_sweep_any(s, ang);
+    const int slot = ang ? 1 : 0;
+    if (s->graph[slot] && s->graph_ang[slot] != ang) {
+        (void)hipGraphExecDestroy(s->graph[slot]);
+        s->graph[slot] = nullptr;
+    }
+    if (!s->graph[slot]) {
+        hipGraph_t g = nullptr;
+        T2_HIP(hipStreamBeginCapture(s->stream,
+                                     hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_sweep_any(s, ang);
+        hipError_t e = hipStreamEndCapture(s->stream, &g);
+        if (rc != T2_OK) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return rc;
+        }
+        T2_HIP(e);
+        hipGraphExec_t ex = nullptr;
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        T2_HIP(e);
+        s->graph[slot] = ex;
+        s->graph_ang[slot] = ang;
+    }
+    T2_HIP(hipGraphLaunch(s->graph[slot], s->stream));
+    return T2_OK;
+}
+
+int total_source(t2_sweeper *s, const double *q, const double *phi)
+{
+    if (s->precision == T2_FP64)
+        T2_HIP(t2::launch_total_source<double>(
+            q, s->d_sigs, phi, s->nc, int(s->ng),
+            static_cast<double *>(s->d_src), s->stream));
+    else
+        T2_HIP(t2::launch_total_source<float>(
+            q, s->d_sigs, phi, s->nc, int(s->ng),
+            static_cast<float *>(s->d_src), s->stream));
+    return T2_OK;
+}
+
+int psi_to_phi(t2_sweeper *s, double *phi)
+{
+    if (s->precision == T2_FP64)
+        T2_HIP(t2::launch_psi_to_phi<double>(
+            static_cast<const double *>(s->d_psi), s->d_wt, s->na,
+            s->n_phi(), phi, s->stream));
+    else
+        T2_HIP(t2::launch_psi_to_phi<float>(
+            static_cast<const float *>(s->d_psi), s->d_wt, s->na, s->n_phi(),
+            phi, s->stream));
+    return T2_OK;
+}
+
+int zero_psi(t2_sweeper *s)
+{
+    // psi is a whole number of doubles in both precisions (4 values/item)
+    const int64_t n = int64_t(s->n_psi() * s->tsize() / 8);
+    T2_HIP(t2::launch_fill(static_cast<double *>(s->d_psi), 0.0, n,
+                           s->stream));
+    return T2_OK;
+}
+
+int fetch(t2_sweeper *s, double *host, const double *dev, int64_t n)
+{
+    T2_HIP(hipMemcpyAsync(host, dev, sizeof(double) * n,
+                          hipMemcpyDeviceToHost, s->stream));
+    T2_HIP(hipStreamSynchronize(s->stream));
+    return T2_OK;
+}
+
+int dot(t2_sweeper *s, const double *x, const double *y, int64_t n,
+        double *out)
+{
+    T2_HIP(t2::launch_multi_dot(x, n, 1, y, n, s->d_red, s->d_scal,
+                                s->stream));
+    return fetch(s, out, s->d_scal, 1);
+}
+
+int check(const t2_sweeper *s)
+{
+    if (!s)
+        return fail(T2_ERR_ARG, "null sweeper handle");
+    return set_device(s);
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+int t2_sweeper_create(t2_sweeper **out, int device, int64_t n_cells,
+                      int64_t n_nodes, const double *coords,
+                      const int64_t *cell_nodes, const int64_t *adj_cell,
+                      const int64_t *f2c, int64_t n_angles,
+                      const double *omega, const double *weights,
+                      int64_t n_groups, const double *sigma_t,
+                      const double *sigma_s, int precision, int gauss)
+{
+    t2::clear_error();
+    if (!out)
+        return fail(T2_ERR_ARG, "t2_sweeper_create: out is null");
+    *out = nullptr;
+    if (n_cells <= 0 || n_nodes <= 0 || n_angles <= 0 || !coords ||
+        !cell_nodes || !adj_cell || !f2c || !omega || !weights || !sigma_t ||
+        !sigma_s)
+        return fail(T2_ERR_ARG, "t2_sweeper_create: bad argument");
+    if (n_groups != 1 && n_groups != 2 && n_groups != 4 && n_groups != 8 &&
+        n_groups != 16 && n_groups != 32 && n_groups != 64)
+        return fail(T2_ERR_ARG, "n_groups must be a power of two <= 64");
+    if (precision != T2_FP64 && precision != T2_FP32)
+        return fail(T2_ERR_ARG, "precision must be T2_FP64 or T2_FP32");
+    if (gauss != T2_GAUSS_NOPIVOT)
+        return fail(T2_ERR_ARG, "only GaussElim NoPivot is implemented");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(T2_ERR_NODEV, "no HIP device");
+    if (device < 0 || device >= ndev)
+        return fail(T2_ERR_ARG, "device index out of range");
+
+    t2::HostMesh hm;
+    T2_TRY(t2::build_host_mesh(n_cells, n_nodes, coords, cell_nodes, adj_cell,
+                               f2c, hm));
+    auto *s = new (std::nothrow) t2_sweeper();
+    if (!s)
+        return fail(T2_ERR_NOMEM, "host allocation");
+    s->device = device;
+    s->nc = n_cells;
+    s->na = n_angles;
+    s->ng = n_groups;
+    s->precision = precision;
+    s->gauss = gauss;
+    if (const char *e = std::getenv("T2_NO_GRAPH"))
+        s->use_graph = !(e[0] == '1');
+    int rc = t2::build_schedule(hm, n_angles, omega, s->sched);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    auto bail = [&](int code) {
+        t2_sweeper_destroy(s);
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) !=
+            hipSuccess) {
+        delete s;
+        return fail(T2_ERR_HIP, "stream creation failed");
+    }
+    const int64_t nc = n_cells, na = n_angles, ng = n_groups;
+    const size_t ts = s->tsize();
+
+    // geometry records
+    std::vector<double> geo(size_t(nc) * t2::kGeoStride, 0.0);
+    std::vector<int4> adj(nc);
+    for (int64_t c = 0; c < nc; c++) {
+        double *g = &geo[size_t(c) * t2::kGeoStride];
+        for (int i = 0; i < 12; i++)
+            g[t2::kGeoNormal + i] = hm.normal[12 * c + i];
+        for (int k = 0; k < 4; k++)
+            g[t2::kGeoArea + k] = hm.area[4 * c + k];
+        g[t2::kGeoVolume] = hm.volume[c];
+        adj[c] = make_int4(int(hm.adj[4 * c]), int(hm.adj[4 * c + 1]),
+                           int(hm.adj[4 * c + 2]), int(hm.adj[4 * c + 3]));
+    }
+    std::vector<char> sigt(size_t(nc) * ng * ts);
+    for (int64_t i = 0; i < nc * ng; i++) {
+        if (precision == T2_FP64)
+            reinterpret_cast<double *>(sigt.data())[i] = sigma_t[i];
+        else
+            reinterpret_cast<float *>(sigt.data())[i] = float(sigma_t[i]);
+    }
+    const int64_t n_phi = nc * 4 * ng;
+    if ((rc = dalloc_t(s, &s->d_omega, sizeof(double) * 3 * na)) ||
+        (rc = dalloc_t(s, &s->d_wt, sizeof(double) * na)) ||
+        (rc = dalloc_t(s, &s->d_geo, sizeof(double) * geo.size())) ||
+        (rc = dalloc_t(s, &s->d_sigs, sizeof(double) * nc * ng)) ||
+        (rc = dalloc(s, &s->d_sigt, sigt.size())) ||
+        (rc = dalloc_t(s, &s->d_adj, sizeof(int4) * nc)) ||
+        (rc = dalloc_t(s, &s->d_nbr, sizeof(uint32_t) * nc)) ||
+        (rc = dalloc_t(s, &s->d_work, sizeof(uint32_t) * s->sched.work.size())) ||
+        (rc = dalloc(s, &s->d_psi, ts * size_t(na) * n_phi)) ||
+        (rc = dalloc(s, &s->d_src, ts * size_t(n_phi))) ||
+        (rc = dalloc_t(s, &s->d_phi, sizeof(double) * n_phi)) ||
+        (rc = dalloc_t(s, &s->d_phi_old, sizeof(double) * n_phi)) ||
+        (rc = dalloc_t(s, &s->d_red, sizeof(double) * t2::kRedBlocks * 64)) ||
+        (rc = dalloc_t(s, &s->d_scal, sizeof(double) * 64)))
+        return bail(rc);
+    const int64_t n_lag = int64_t(s->sched.lagged.size());
+    if (n_lag > 0) {
+        std::vector<int64_t> key(n_lag);
+        std::vector<int32_t> up(n_lag);
+        for (int64_t i = 0; i < n_lag; i++) {
+            const auto &l = s->sched.lagged[i];
+            key[i] = (int64_t(l.angle) * nc + l.cell) * 4 + l.face;
+            up[i] = l.upwind;
+        }
+        if ((rc = dalloc_t(s, &s->d_lag_key, sizeof(int64_t) * n_lag)) ||
+            (rc = dalloc_t(s, &s->d_lag_up, sizeof(int32_t) * n_lag)) ||
+            (rc = dalloc(s, &s->d_lag_buf, ts * size_t(n_lag) * 4 * ng)))
+            return bail(rc);
+        if (hipMemcpy(s->d_lag_key, key.data(), sizeof(int64_t) * n_lag,
+                      hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s->d_lag_up, up.data(), sizeof(int32_t) * n_lag,
+                      hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(T2_ERR_HIP, "upload of lagged faces failed"));
+    }
+    if (hipMemcpy(s->d_omega, omega, sizeof(double) * 3 * na,
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_wt, weights, sizeof(double) * na,
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_geo, geo.data(), sizeof(double) * geo.size(),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_sigs, sigma_s, sizeof(double) * nc * ng,
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_sigt, sigt.data(), sigt.size(),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_adj, adj.data(), sizeof(int4) * nc,
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_nbr, hm.nbr_pack.data(), sizeof(uint32_t) * nc,
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_work, s->sched.work.data(),
+                  sizeof(uint32_t) * s->sched.work.size(),
+                  hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(T2_ERR_HIP, "upload of set-up data failed"));
+    std::vector<uint32_t>().swap(s->sched.work);  // device copy is enough
+    if ((rc = zero_psi(s)) ||
+        (rc = total_source(s, nullptr, nullptr)))
+        return bail(rc);
+    if (hipMemsetAsync(s->d_phi, 0, sizeof(double) * n_phi, s->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return bail(fail(T2_ERR_HIP, "initialisation failed"));
+    *out = s;
+    return T2_OK;
+}
+
+int t2_sweeper_destroy(t2_sweeper *s)
+{
+    if (!s)
+        return T2_OK;
+    (void)hipSetDevice(s->device);
+    if (s->stream)
+        (void)hipStreamSynchronize(s->stream);
+    drop_graphs(s);
+    for (void *p : s->allocs)
+        (void)hipFree(p);
+    if (s->stream)
+        (void)hipStreamDestroy(s->stream);
+    delete s;
+    return T2_OK;
+}
+
+int t2_sweeper_info(const t2_sweeper *s, t2_info *info)
+{
+    t2::clear_error();
+    if (!s || !info)
+        return fail(T2_ERR_ARG, "t2_sweeper_info: bad argument");
+    info->n_cells = s->nc;
+    info->n_angles = s->na;
+    info->n_groups = s->ng;
+    info->n_steps = s->sched.n_steps;
+    info->min_levels = s->sched.min_levels;
+    info->n_lagged = int64_t(s->sched.lagged.size());
+    info->psi_bytes = int64_t(s->n_psi() * s->tsize());
+    info->device_bytes = int64_t(s->bytes);
+    info->precision = s->precision;
+    info->gauss = s->gauss;
+    return T2_OK;
+}
+
+int t2_set_launch(t2_sweeper *s, int items_per_block, int groups_per_lane)
+{
+    t2::clear_error();
+    if (!s)
+        return fail(T2_ERR_ARG, "null sweeper handle");
+    int gpl = groups_per_lane ? groups_per_lane : s->gpl;
+    if (gpl != 1 && gpl != 2 && gpl != 4)
+        return fail(T2_ERR_ARG, "groups_per_lane must be 1, 2 or 4");
+    if (gpl > s->ng)
+        return fail(T2_ERR_ARG, "groups_per_lane exceeds the group count");
+    int block = s->block;
+    if (items_per_block) {
+        const int64_t lanes = int64_t(items_per_block) * (s->ng / gpl);
+        if (lanes < 64 || lanes > t2::kMaxBlock || lanes % 64)
+            return fail(T2_ERR_ARG,
+                        "items_per_block x lanes per item must be a "
+                        "multiple of 64 in [64, 256]");
+        block = int(lanes);
+    }
+    if (block != s->block || gpl != s->gpl)
+        drop_graphs(s);
+    s->block = block;
+    s->gpl = gpl;
+    return T2_OK;
+}
+
+int t2_sweep_host(t2_sweeper *s, double *psi, const double *source,
+                  int zero_psi_bound)
+{
+    (void)zero_psi_bound;  // no partition faces on one device
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (!psi || !source)
+        return fail(T2_ERR_ARG, "t2_sweep_host: null array");
+    const int64_t n = s->n_psi();
+    const int64_t vg = 4 * s->ng;
+    if (!s->d_stage)
+        T2_TRY(dalloc_t(s, &s->d_stage, sizeof(double) * n));
+    if (!s->d_hostang)
+        T2_TRY(dalloc_t(s, &s->d_hostang, sizeof(double) * n));
+    T2_HIP(hipMemcpyAsync(s->d_stage, source, sizeof(double) * n,
+                          hipMemcpyHostToDevice, s->stream));
+    T2_HIP(t2::launch_ca_to_ac(s->d_stage, s->d_hostang, s->nc, s->na, vg,
+                               s->stream));
+    if (!s->sched.lagged.empty()) {
+        // lagged faces read the caller's previous psi
+        T2_HIP(hipMemcpyAsync(s->d_stage, psi, sizeof(double) * n,
+                              hipMemcpyHostToDevice, s->stream));
+        if (s->precision == T2_FP64) {
+            T2_HIP(t2::launch_ca_to_ac(s->d_stage,
+                                       static_cast<double *>(s->d_psi), s->nc,
+                                       s->na, vg, s->stream));
+        } else {
+            return fail(T2_ERR_STATE,
+                        "host sweep with lagged faces needs FP64 storage");
+        }
+    }
+    T2_TRY(total_source(s, nullptr, nullptr));
+    T2_TRY(run_sweep(s, s->d_hostang));
+    if (s->precision == T2_FP64)
+        T2_HIP(t2::launch_ac_to_ca<double>(
+            static_cast<const double *>(s->d_psi), s->d_stage, s->nc, s->na,
+            vg, s->stream));
+    else
+        T2_HIP(t2::launch_ac_to_ca<float>(static_cast<const float *>(s->d_psi),
+                                          s->d_stage, s->nc, s->na, vg,
+                                          s->stream));
+    return fetch(s, psi, s->d_stage, n);
+}
+
+int t2_set_source_isotropic(t2_sweeper *s, const double *q)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    T2_HIP(hipStreamSynchronize(s->stream));
+    if (s->d_qang) {
+        dfree(s, s->d_qang, sizeof(double) * s->n_psi());
+        s->d_qang = nullptr;
+        drop_graphs(s);
+    }
+    if (!q) {
+        if (s->d_qiso)
+            dfree(s, s->d_qiso, sizeof(double) * s->n_phi());
+        s->d_qiso = nullptr;
+        return T2_OK;
+    }
+    if (!s->d_qiso)
+        T2_TRY(dalloc_t(s, &s->d_qiso, sizeof(double) * s->n_phi()));
+    T2_HIP(hipMemcpy(s->d_qiso, q, sizeof(double) * s->n_phi(),
+                     hipMemcpyHostToDevice));
+    return T2_OK;
+}
+
+int t2_set_source_angular(t2_sweeper *s, const double *source)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    T2_TRY(t2_set_source_isotropic(s, nullptr));
+    if (!source)
+        return T2_OK;
+    const int64_t n = s->n_psi();
+    T2_TRY(dalloc_t(s, &s->d_qang, sizeof(double) * n));
+    if (!s->d_stage)
+        T2_TRY(dalloc_t(s, &s->d_stage, sizeof(double) * n));
+    T2_HIP(hipMemcpyAsync(s->d_stage, source, sizeof(double) * n,
+                          hipMemcpyHostToDevice, s->stream));
+    T2_HIP(t2::launch_ca_to_ac(s->d_stage, s->d_qang, s->nc, s->na,
+                               4 * s->ng, s->stream));
+    T2_HIP(hipStreamSynchronize(s->stream));
+    return T2_OK;
+}
+
+int t2_sweep_step(t2_sweeper *s)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    T2_TRY(total_source(s, s->d_qiso, s->d_phi));
+    T2_TRY(run_sweep(s, s->d_qang));
+    std::swap(s->d_phi, s->d_phi_old);
+    T2_TRY(psi_to_phi(s, s->d_phi));
+    return T2_OK;
+}
+
+int t2_solve_source_iteration(t2_sweeper *s, int64_t iter_max, double err_max,
+                              int64_t *n_iter, double *err_hist)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (iter_max < 0)
+        return fail(T2_ERR_ARG, "iter_max < 0");
+    T2_TRY(zero_psi(s));
+    T2_HIP(t2::launch_fill(s->d_phi, 0.0, s->n_phi(), s->stream));
+    int64_t it = 0;
+    double err = 1.0;
+    while (it < iter_max && err > err_max) {
+        T2_TRY(t2_sweep_step(s));
+        T2_HIP(t2::launch_l1_change(s->d_phi, s->d_phi_old, s->n_phi(),
+                                    s->d_red, s->d_scal, s->stream));
+        double sums[2];
+        T2_TRY(fetch(s, sums, s->d_scal, 2));
+        err = sums[0] / sums[1];
+        if (err_hist)
+            err_hist[it] = err;
+        it++;
+    }
+    if (n_iter)
+        *n_iter = it;
+    return T2_OK;
+}
+
+int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,
+                    int64_t restart, int64_t *n_iter, double *res_hist)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (restart < 1 || restart > 60 || iter_max < 0)
+        return fail(T2_ERR_ARG, "restart must be in [1, 60]");
+    const int64_t n = s->n_phi();
+    const int64_t m = restart;
+    if (s->kry_cols < m + 2) {
+        if (s->d_kry)
+            dfree(s, s->d_kry, sizeof(double) * n * s->kry_cols);
+        s->d_kry = nullptr;
+        s->kry_cols = 0;
+        T2_TRY(dalloc_t(s, &s->d_kry, sizeof(double) * n * (m + 2)));
+        s->kry_cols = m + 2;
+    }
+    double *V = s->d_kry;              // m + 1 basis vectors
+    double *b = s->d_kry + n * (m + 1);
+    double *x = s->d_phi;
+    T2_TRY(zero_psi(s));
+
+    // b = D L^-1 q
+    T2_TRY(total_source(s, s->d_qiso, nullptr));
+    T2_TRY(run_sweep(s, s->d_qang));
+    T2_TRY(psi_to_phi(s, b));
+    double bb = 0.0;
+    T2_TRY(dot(s, b, b, n, &bb));
+    const double bnorm = std::sqrt(bb), target = rtol * bnorm;
+    T2_HIP(t2::launch_fill(x, 0.0, n, s->stream));
+
+    // y = (I - D L^-1 M S) v
+    auto apply = [&](const double *v, double *y) -> int {
+        T2_TRY(total_source(s, nullptr, v));
+        T2_TRY(run_sweep(s, nullptr));
+        T2_TRY(psi_to_phi(s, y));
+        T2_HIP(t2::launch_axpby(1.0, v, -1.0, y, n, s->stream));
+        return T2_OK;
+    };
+
+    std::vector<double> H(size_t(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1),
+        y(m), h(m + 1), h2(m + 1);
+    int64_t it = 0;
+    double rnorm = bnorm;
+    while (it < iter_max && rnorm > target) {
+        T2_TRY(apply(x, V));                                  // V0 = A x
+        T2_HIP(t2::launch_axpby(1.0, b, -1.0, V, n, s->stream));  // b - A x
+        double beta2 = 0.0;
+        T2_TRY(dot(s, V, V, n, &beta2));
+        const double beta = std::sqrt(beta2);
+        rnorm = beta;
+        if (beta <= target || beta == 0.0)
+            break;
+        T2_HIP(t2::launch_axpby(0.0, V, 1.0 / beta, V, n, s->stream));
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int64_t j = 0;
+        for (; j < m && it < iter_max; j++) {
+            double *w = V + n * (j + 1);
+            T2_TRY(apply(V + n * j, w));
+            // classical Gram-Schmidt, applied twice, in FP64
+            for (int pass = 0; pass < 2; pass++) {
+                T2_HIP(t2::launch_multi_dot(V, n, int(j + 1), w, n, s->d_red,
+                                            s->d_scal, s->stream));
+                T2_HIP(t2::launch_multi_axpy(V, n, int(j + 1), s->d_scal, w,
+                                             n, s->stream));
+                T2_TRY(fetch(s, pass ? h2.data() : h.data(), s->d_scal,
+                             j + 1));
+            }
+            for (int64_t i = 0; i <= j; i++)
+                H[i * m + j] = h[i] + h2[i];
+            double ww = 0.0;
+            T2_TRY(dot(s, w, w, n, &ww));
+            const double hn = std::sqrt(ww);
+            H[(j + 1) * m + j] = hn;
+            if (hn != 0.0)
+                T2_HIP(t2::launch_axpby(0.0, w, 1.0 / hn, w, n, s->stream));
+            for (int64_t i = 0; i < j; i++) {
+                const double t = cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
+                H[(i + 1) * m + j] =
+                    -sn[i] * H[i * m + j] + cs[i] * H[(i + 1) * m + j];
+                H[i * m + j] = t;
+            }
+            const double hjj = H[j * m + j], hj1 = H[(j + 1) * m + j];
+            const double r = std::sqrt(hjj * hjj + hj1 * hj1);
+            cs[j] = hjj / r;
+            sn[j] = hj1 / r;
+            H[j * m + j] = r;
+            H[(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            rnorm = std::fabs(g[j + 1]);
+            if (res_hist)
+                res_hist[it] = rnorm / (bnorm > 0 ? bnorm : 1.0);
+            it++;
+            if (rnorm <= target) {
+                j++;
+                break;
+            }
+        }
+        for (int64_t i = j - 1; i >= 0; i--) {
+            double acc = g[i];
+            for (int64_t k = i + 1; k < j; k++)
+                acc -= H[i * m + k] * y[k];
+            y[i] = acc / H[i * m + i];
+        }
+        for (int64_t i = 0; i < j; i++)
+            y[i] = -y[i];
+        T2_HIP(hipMemcpyAsync(s->d_scal, y.data(), sizeof(double) * j,
+                              hipMemcpyHostToDevice, s->stream));
+        T2_HIP(t2::launch_multi_axpy(V, n, int(j), s->d_scal, x, n,
+                                     s->stream));
+        T2_HIP(hipStreamSynchronize(s->stream));
+    }
+    // psi = L^-1 (q + M S x)
+    T2_TRY(total_source(s, s->d_qiso, x));
+    T2_TRY(run_sweep(s, s->d_qang));
+    T2_HIP(hipStreamSynchronize(s->stream));
+    if (n_iter)
+        *n_iter = it;
+    return T2_OK;
+}
+
+int t2_get_phi(const t2_sweeper *s, double *phi)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (!phi)
+        return fail(T2_ERR_ARG, "null phi");
+    T2_HIP(hipStreamSynchronize(s->stream));
+    T2_HIP(hipMemcpy(phi, s->d_phi, sizeof(double) * s->n_phi(),
+                     hipMemcpyDeviceToHost));
+    return T2_OK;
+}
+
+int t2_set_phi(t2_sweeper *s, const double *phi)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (!phi)
+        return fail(T2_ERR_ARG, "null phi");
+    T2_HIP(hipStreamSynchronize(s->stream));
+    T2_HIP(hipMemcpy(s->d_phi, phi, sizeof(double) * s->n_phi(),
+                     hipMemcpyHostToDevice));
+    return T2_OK;
+}
+
+int t2_get_psi(const t2_sweeper *s_const, double *psi)
+{
+    t2::clear_error();
+    t2_sweeper *s = const_cast<t2_sweeper *>(s_const);
+    T2_TRY(check(s));
+    if (!psi)
+        return fail(T2_ERR_ARG, "null psi");
+    const int64_t n = s->n_psi();
+    if (!s->d_stage)
+        T2_TRY(dalloc_t(s, &s->d_stage, sizeof(double) * n));
+    if (s->precision == T2_FP64)
+        T2_HIP(t2::launch_ac_to_ca<double>(
+            static_cast<const double *>(s->d_psi), s->d_stage, s->nc, s->na,
+            4 * s->ng, s->stream));
+    else
+        T2_HIP(t2::launch_ac_to_ca<float>(static_cast<const float *>(s->d_psi),
+                                          s->d_stage, s->nc, s->na, 4 * s->ng,
+                                          s->stream));
+    return fetch(s, psi, s->d_stage, n);
+}
+
+int t2_get_psi_angle(const t2_sweeper *s, int64_t angle, double *psi_a)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (!psi_a || angle < 0 || angle >= s->na)
+        return fail(T2_ERR_ARG, "t2_get_psi_angle: bad argument");
+    const int64_t n = s->n_phi();
+    T2_HIP(hipStreamSynchronize(s->stream));
+    if (s->precision == T2_FP64) {
+        T2_HIP(hipMemcpy(psi_a, static_cast<const double *>(s->d_psi) + angle * n,
+                         sizeof(double) * n, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(n);
+        T2_HIP(hipMemcpy(tmp.data(),
+                         static_cast<const float *>(s->d_psi) + angle * n,
+                         sizeof(float) * n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; i++)
+            psi_a[i] = double(tmp[i]);
+    }
+    return T2_OK;
+}
+
+int t2_synchronize(t2_sweeper *s)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    T2_HIP(hipStreamSynchronize(s->stream));
+    return T2_OK;
+}
+
+int t2_get_stream(const t2_sweeper *s, void **stream)
+{
+    t2::clear_error();
+    if (!s || !stream)
+        return fail(T2_ERR_ARG, "t2_get_stream: bad argument");
+    *stream = static_cast<void *>(s->stream);
+    return T2_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,75 @@
+// Shared host-side declarations of the sweep engine.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tycho2_sweep.h"
+
+namespace t2 {
+
+// Records the failure message for t2_last_error() and returns `code`.
+int fail(int code, const std::string &msg);
+void clear_error();
+
+// Per-cell geometry and connectivity in the engine's host layout.
+struct HostMesh {
+    int64_t nc = 0;
+    std::vector<double> volume;    // [nc]
+    std::vector<double> area;      // [nc][4]
+    std::vector<double> normal;    // [nc][4][3]
+    std::vector<int64_t> adj;      // [nc][4], -1 on the boundary
+    // neighbour vertex through face k at cell vertex j (j != k), two bits at
+    // position 2*(4k+j); the j == k slots are zero.
+    std::vector<uint32_t> nbr_pack;
+};
+
+// Geometry of the reference's TychoMesh constructor.
+int mesh_geometry(int64_t nc, int64_t nn, const double *coords,
+                  const int64_t *cell_nodes, const int64_t *f2c,
+                  double *volume, double *area, double *normal);
+
+int build_host_mesh(int64_t nc, int64_t nn, const double *coords,
+                    const int64_t *cell_nodes, const int64_t *adj,
+                    const int64_t *f2c, HostMesh &m);
+
+// Omega . n with every product and sum rounded separately (this file is
+// built with -ffp-contract=off; the device evaluates the same expression
+// with explicitly rounded operations, so both sides agree on which faces are
+// incoming).
+inline double omega_dot_n(const double *om, const double *n)
+{
+    return om[0] * n[0] + om[1] * n[1] + om[2] * n[2];
+}
+
+struct LaggedFace {
+    int32_t angle;
+    int32_t cell;
+    int32_t face;
+    int32_t upwind;
+};
+
+// Sweep schedule: every (angle, cell) pair in level order.
+struct Schedule {
+    int64_t n_steps = 0;
+    int64_t min_levels = 0;
+    int cell_bits = 0;                 // work item = cell | angle << cell_bits
+    std::vector<int64_t> step_off;     // [n_steps + 1]
+    std::vector<uint32_t> work;        // [nc * na]
+    std::vector<LaggedFace> lagged;    // sorted by (angle, cell, face)
+};
+
+// Levels of one angle; returns the lagged faces appended to `lagged`.
+void angle_levels(const HostMesh &m, const double *om, int32_t angle,
+                  int32_t *level, std::vector<LaggedFace> &lagged,
+                  std::vector<int32_t> &scratch_indeg,
+                  std::vector<int32_t> &scratch_frontier);
+
+int build_levels(const HostMesh &m, int64_t na, const double *omega,
+                 int32_t *levels, std::vector<LaggedFace> &lagged);
+
+int build_schedule(const HostMesh &m, int64_t na, const double *omega,
+                   Schedule &s);
+
+}  // namespace t2

@@ -1,0 +1,592 @@
+// Device kernels of the sweep engine (gfx950).
+//
+// The cell solve restates the reference's linear-DG update of one
+// (cell, angle) for all groups (src/SweepData.hh:139-174,
+// src/Transport.cc:54-410, 687-776) in registers: one work item is one
+// (angle, cell); the lanes of a wavefront cover consecutive groups of an
+// item, so every flux load and store touches group-contiguous memory.
+#include "t2_kernels.hpp"
+
+namespace t2 {
+namespace {
+
+constexpr double kInv12 = 1.0 / 12.0;
+constexpr double kInv20 = 1.0 / 20.0;
+
+// Omega . n with each product and sum rounded separately: the host level
+// builder evaluates the same expression, so both agree on face directions.
+__device__ __forceinline__ double omega_dot_n(double ox, double oy, double oz,
+                                              const double *n)
+{
+#pragma clang fp contract(off)
+    return ox * n[0] + oy * n[1] + oz * n[2];
+}
+
+template <typename T>
+__device__ __forceinline__ T recip(T x);
+template <>
+__device__ __forceinline__ double recip<double>(double x)
+{
+    return 1.0 / x;
+}
+template <>
+__device__ __forceinline__ float recip<float>(float x)
+{
+    return __builtin_amdgcn_rcpf(x);
+}
+
+// Index of `key` in the sorted array, or -1.
+__device__ __forceinline__ int64_t find_key(const int64_t *keys, int64_t n,
+                                            int64_t key)
+{
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return (lo < n && keys[lo] == key) ? lo : -1;
+}
+
+// 4x4 solve by row elimination without pivoting (src/Transport.cc:338-410):
+// rows 0..2 scaled by their pivot's reciprocal, last pivot divides.
+template <typename T>
+__device__ __forceinline__ void solve_nopivot(T m[4][4], T b[4])
+{
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const T inv = recip<T>(m[r][r]);
+#pragma unroll
+        for (int c = r + 1; c < 4; c++)
+            m[r][c] *= inv;
+        b[r] *= inv;
+#pragma unroll
+        for (int row = r + 1; row < 4; row++) {
+            const T t = m[row][r];
+#pragma unroll
+            for (int c = r + 1; c < 4; c++)
+                m[row][c] -= m[r][c] * t;
+            b[row] -= b[r] * t;
+        }
+    }
+    b[3] = b[3] / m[3][3];
+#pragma unroll
+    for (int r = 2; r >= 0; r--)
+#pragma unroll
+        for (int c = 3; c > r; c--)
+            b[r] -= m[r][c] * b[c];
+}
+
+template <typename T, int G, int GPL>
+__global__ void __launch_bounds__(kMaxBlock)
+sweep_step_kernel(const SweepArgs<T> p)
+{
+    constexpr int LPI = G / GPL;  // lanes per work item
+    const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t item = tid / LPI;
+    const int gl = int(tid % LPI);
+    if (item >= p.n_items)
+        return;
+    const uint32_t w = p.work[item];
+    const int64_t c = int64_t(w & ((1u << p.cell_bits) - 1u));
+    const int64_t a = int64_t(w >> p.cell_bits);
+    const double ox = p.omega[3 * a], oy = p.omega[3 * a + 1],
+                 oz = p.omega[3 * a + 2];
+    const double *geo = p.geo + c * kGeoStride;
+
+    // Abar_k = A_k (Omega . n_k) (src/Transport.cc:697-704); the tests below
+    // are the reference's incoming / outgoing rules (Transport.cc:113,184).
+    bool in_k[4], out_k[4];
+    T a12[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double abar =
+            geo[kGeoArea + k] * omega_dot_n(ox, oy, oz, geo + 3 * k);
+        in_k[k] = abar < 0.0;
+        out_k[k] = abar > 0.0;
+        a12[k] = T(abar * kInv12);
+    }
+    const T v20 = T(geo[kGeoVolume] * kInv20);
+    const int4 nb4 = p.adj[c];
+    const int32_t nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    const uint32_t np = p.nbr[c];
+
+    // outgoing-face sums of the streaming matrix
+    T P = T(0);
+    T o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        o[k] = out_k[k] ? a12[k] : T(0);
+        P += o[k];
+    }
+
+    const size_t plane = size_t(p.nc) * 4 * G;  // one angle of psi
+    const T *psi_a = p.psi + size_t(a) * plane;
+    int64_t lag_at[4] = {-1, -1, -1, -1};
+    if (p.n_lag > 0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (in_k[k] && nb[k] >= 0)
+                lag_at[k] = find_key(p.lag_key, p.n_lag, (a * p.nc + c) * 4 + k);
+    }
+
+#pragma unroll
+    for (int jg = 0; jg < GPL; jg++) {
+        const int g = gl + jg * LPI;
+        const T s = p.sigma_t[c * G + g] * v20;
+
+        // source: V/20 (q_i + sum_j q_j)  (Transport.cc:58-72)
+        T q[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            q[v] = p.src_iso[(c * 4 + v) * G + g];
+        if (p.src_ang) {
+            const double *qa = p.src_ang + size_t(a) * plane;
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                q[v] += T(qa[(c * 4 + v) * G + g]);
+        }
+        const T sumq = (q[0] + q[1]) + (q[2] + q[3]);
+        T b[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            b[v] = v20 * (sumq + q[v]);
+
+        // upwind faces: b_i -= a_k (sum_j psi_j + psi_i), j, i != k
+        // (Transport.cc:175-271, 740-776); vacuum faces contribute zero
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!in_k[k] || nb[k] < 0)
+                continue;
+            T u[4];
+            if (lag_at[k] < 0) {
+                const T *pn = psi_a + size_t(nb[k]) * 4 * G + g;
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    u[j] = (j == k) ? T(0)
+                                    : pn[((np >> (2 * (4 * k + j))) & 3u) * G];
+            } else {
+                const T *pl = p.lag_buf + size_t(lag_at[k]) * 4 * G + g;
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    u[j] = (j == k) ? T(0) : pl[j * G];
+            }
+            const T sum = (u[0] + u[1]) + (u[2] + u[3]);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (i != k)
+                    b[i] -= a12[k] * (sum + u[i]);
+        }
+
+        // M[i][j] = a_i + (1 + d_ij)(s + sum_{k out, k != i, j} a_k)
+        // (Transport.cc:79-168)
+        T m[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                m[i][j] = (i == j) ? a12[i] + T(2) * (s + (P - o[i]))
+                                   : a12[i] + s + ((P - o[i]) - o[j]);
+        solve_nopivot<T>(m, b);
+
+        T *dst = p.psi + size_t(a) * plane + size_t(c) * 4 * G + g;
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            dst[v * G] = b[v];
+    }
+}
+
+template <typename T, int G>
+hipError_t launch_g(const SweepArgs<T> &args, int gpl, int block,
+                    hipStream_t stream)
+{
+    if (block < 64 || block > kMaxBlock || block % 64)
+        return hipErrorInvalidConfiguration;
+    if (args.n_items <= 0)
+        return hipSuccess;
+    const int64_t lanes_per_item = G / gpl;
+    const int64_t threads = args.n_items * lanes_per_item;
+    const unsigned grid = unsigned((threads + block - 1) / block);
+    switch (gpl) {
+    case 1:
+        hipLaunchKernelGGL((sweep_step_kernel<T, G, 1>), dim3(grid),
+                           dim3(block), 0, stream, args);
+        break;
+    case 2:
+        if constexpr (G >= 2) {
+            hipLaunchKernelGGL((sweep_step_kernel<T, G, 2>), dim3(grid),
+                               dim3(block), 0, stream, args);
+            break;
+        }
+        return hipErrorInvalidValue;
+    case 4:
+        if constexpr (G >= 4) {
+            hipLaunchKernelGGL((sweep_step_kernel<T, G, 4>), dim3(grid),
+                               dim3(block), 0, stream, args);
+            break;
+        }
+        return hipErrorInvalidValue;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- lagging
+template <typename T>
+__global__ void lag_gather_kernel(const int64_t *lag_key,
+                                  const int32_t *lag_up, int64_t n_lag,
+                                  int64_t nc, int G, const uint32_t *nbr,
+                                  const T *psi, T *lag_buf)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_lag * 4 * G)
+        return;
+    const int g = int(i % G);
+    const int j = int((i / G) % 4);
+    const int64_t l = i / (4 * G);
+    const int64_t key = lag_key[l];
+    const int k = int(key & 3);
+    const int64_t ac = key >> 2;
+    const int64_t c = ac % nc, a = ac / nc;
+    T v = T(0);
+    if (j != k) {
+        const int vtx = int((nbr[c] >> (2 * (4 * k + j))) & 3u);
+        v = psi[((size_t(a) * nc + lag_up[l]) * 4 + vtx) * G + g];
+    }
+    lag_buf[i] = v;
+}
+
+// ---------------------------------------------------------------- moments
+template <typename T>
+__global__ void psi_to_phi_kernel(const T *__restrict__ psi,
+                                  const double *__restrict__ wt, int64_t na,
+                                  int64_t n_phi, double *__restrict__ phi)
+{
+#pragma clang fp contract(off)
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+         i < n_phi; i += int64_t(gridDim.x) * blockDim.x) {
+        double acc = 0.0;
+        for (int64_t a = 0; a < na; a++)
+            acc += double(psi[size_t(a) * n_phi + i]) * wt[a];
+        phi[i] = acc;
+    }
+}
+
+template <typename T>
+__global__ void total_source_kernel(const double *__restrict__ q,
+                                    const double *__restrict__ sigma_s,
+                                    const double *__restrict__ phi,
+                                    int64_t nc, int G, T *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int64_t n = nc * 4 * G;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t c = i / (4 * G);
+        const int g = int(i % G);
+        const double s = sigma_s[c * G + g] / (4.0 * 3.14159265358979323846);
+        const double v = (q ? q[i] : 0.0) + s * (phi ? phi[i] : 0.0);
+        out[i] = T(v);
+    }
+}
+
+__global__ void ca_to_ac_kernel(const double *__restrict__ in,
+                                double *__restrict__ out, int64_t nc,
+                                int64_t na, int64_t vg)
+{
+    const int64_t n = nc * na * vg;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t e = i % vg;
+        const int64_t ca = i / vg;   // output index (a, c)
+        const int64_t c = ca % nc, a = ca / nc;
+        out[i] = in[(c * na + a) * vg + e];
+    }
+}
+
+template <typename T>
+__global__ void ac_to_ca_kernel(const T *__restrict__ in,
+                                double *__restrict__ out, int64_t nc,
+                                int64_t na, int64_t vg)
+{
+    const int64_t n = nc * na * vg;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t e = i % vg;
+        const int64_t ca = i / vg;   // output index (c, a)
+        const int64_t a = ca % na, c = ca / na;
+        out[i] = double(in[(a * nc + c) * vg + e]);
+    }
+}
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ double block_sum(double v, double *lds)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        lds[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < int(blockDim.x >> 6); i++)
+            s += lds[i];
+    __syncthreads();
+    return s;  // valid in thread 0
+}
+
+__global__ void l1_change_kernel(const double *__restrict__ x,
+                                 const double *__restrict__ y, int64_t n,
+                                 double *partial)
+{
+    __shared__ double lds[16];
+    double s0 = 0.0, s1 = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        s0 += fabs(x[i] - y[i]);
+        s1 += fabs(x[i]);
+    }
+    s0 = block_sum(s0, lds);
+    s1 = block_sum(s1, lds);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s0;
+        partial[gridDim.x + blockIdx.x] = s1;
+    }
+}
+
+// out[r] = sum of partial[r * count .. (r+1) * count) in index order
+__global__ void finish_sums_kernel(const double *partial, int count,
+                                   double *out)
+{
+    __shared__ double lds[16];
+    const double *p = partial + size_t(blockIdx.x) * count;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < count; i += blockDim.x)
+        s += p[i];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0)
+        out[blockIdx.x] = s;
+}
+
+constexpr int kDotChunk = 8;
+
+__global__ void multi_dot_kernel(const double *__restrict__ V, int64_t ldv,
+                                 int m, const double *__restrict__ w,
+                                 int64_t n, double *partial)
+{
+    __shared__ double lds[16];
+    const int j0 = blockIdx.y * kDotChunk;
+    double acc[kDotChunk];
+#pragma unroll
+    for (int j = 0; j < kDotChunk; j++)
+        acc[j] = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const double wi = w[i];
+#pragma unroll
+        for (int j = 0; j < kDotChunk; j++)
+            if (j0 + j < m)
+                acc[j] += V[size_t(j0 + j) * ldv + i] * wi;
+    }
+#pragma unroll
+    for (int j = 0; j < kDotChunk; j++) {
+        const double s = block_sum(acc[j], lds);
+        if (threadIdx.x == 0 && j0 + j < m)
+            partial[size_t(j0 + j) * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+__global__ void multi_axpy_kernel(const double *__restrict__ V, int64_t ldv,
+                                  int m, const double *__restrict__ h,
+                                  double *__restrict__ w, int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        double s = w[i];
+        for (int j = 0; j < m; j++)
+            s -= h[j] * V[size_t(j) * ldv + i];
+        w[i] = s;
+    }
+}
+
+__global__ void axpby_kernel(double alpha, const double *__restrict__ x,
+                             double beta, double *__restrict__ y, int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+        y[i] = alpha * x[i] + beta * y[i];
+}
+
+__global__ void fill_kernel(double *x, double v, int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+        x[i] = v;
+}
+
+inline unsigned stream_grid(int64_t n, int block = 256)
+{
+    int64_t g = (n + block - 1) / block;
+    if (g > 65536)
+        g = 65536;
+    if (g < 1)
+        g = 1;
+    return unsigned(g);
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_sweep_step(const SweepArgs<T> &args, int groups,
+                             int groups_per_lane, int block,
+                             hipStream_t stream)
+{
+    switch (groups) {
+    case 1: return launch_g<T, 1>(args, groups_per_lane, block, stream);
+    case 2: return launch_g<T, 2>(args, groups_per_lane, block, stream);
+    case 4: return launch_g<T, 4>(args, groups_per_lane, block, stream);
+    case 8: return launch_g<T, 8>(args, groups_per_lane, block, stream);
+    case 16: return launch_g<T, 16>(args, groups_per_lane, block, stream);
+    case 32: return launch_g<T, 32>(args, groups_per_lane, block, stream);
+    case 64: return launch_g<T, 64>(args, groups_per_lane, block, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t launch_lag_gather(const int64_t *lag_key, const int32_t *lag_up,
+                             int64_t n_lag, int64_t nc, int groups,
+                             const uint32_t *nbr, const T *psi, T *lag_buf,
+                             hipStream_t stream)
+{
+    if (n_lag <= 0)
+        return hipSuccess;
+    const int64_t n = n_lag * 4 * groups;
+    hipLaunchKernelGGL((lag_gather_kernel<T>), dim3(stream_grid(n)),
+                       dim3(256), 0, stream, lag_key, lag_up, n_lag, nc,
+                       groups, nbr, psi, lag_buf);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_psi_to_phi(const T *psi, const double *wt, int64_t na,
+                             int64_t n_phi, double *phi, hipStream_t stream)
+{
+    hipLaunchKernelGGL((psi_to_phi_kernel<T>), dim3(stream_grid(n_phi)),
+                       dim3(256), 0, stream, psi, wt, na, n_phi, phi);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_total_source(const double *q, const double *sigma_s,
+                               const double *phi, int64_t nc, int groups,
+                               T *out, hipStream_t stream)
+{
+    const int64_t n = nc * 4 * groups;
+    hipLaunchKernelGGL((total_source_kernel<T>), dim3(stream_grid(n)),
+                       dim3(256), 0, stream, q, sigma_s, phi, nc, groups,
+                       out);
+    return hipGetLastError();
+}
+
+hipError_t launch_ca_to_ac(const double *in, double *out, int64_t nc,
+                           int64_t na, int64_t vg, hipStream_t stream)
+{
+    hipLaunchKernelGGL(ca_to_ac_kernel, dim3(stream_grid(nc * na * vg)),
+                       dim3(256), 0, stream, in, out, nc, na, vg);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ac_to_ca(const T *in, double *out, int64_t nc, int64_t na,
+                           int64_t vg, hipStream_t stream)
+{
+    hipLaunchKernelGGL((ac_to_ca_kernel<T>), dim3(stream_grid(nc * na * vg)),
+                       dim3(256), 0, stream, in, out, nc, na, vg);
+    return hipGetLastError();
+}
+
+hipError_t launch_l1_change(const double *x, const double *y, int64_t n,
+                            double *partial, double *out, hipStream_t stream)
+{
+    hipLaunchKernelGGL(l1_change_kernel, dim3(kRedBlocks), dim3(kRedThreads),
+                       0, stream, x, y, n, partial);
+    hipLaunchKernelGGL(finish_sums_kernel, dim3(2), dim3(kRedThreads), 0,
+                       stream, partial, kRedBlocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_dot(const double *V, int64_t ldv, int m,
+                            const double *w, int64_t n, double *partial,
+                            double *out, hipStream_t stream)
+{
+    if (m <= 0)
+        return hipSuccess;
+    const unsigned chunks = unsigned((m + kDotChunk - 1) / kDotChunk);
+    hipLaunchKernelGGL(multi_dot_kernel, dim3(kRedBlocks, chunks),
+                       dim3(kRedThreads), 0, stream, V, ldv, m, w, n,
+                       partial);
+    hipLaunchKernelGGL(finish_sums_kernel, dim3(m), dim3(kRedThreads), 0,
+                       stream, partial, kRedBlocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_axpy(const double *V, int64_t ldv, int m,
+                             const double *h, double *w, int64_t n,
+                             hipStream_t stream)
+{
+    hipLaunchKernelGGL(multi_axpy_kernel, dim3(stream_grid(n)), dim3(256), 0,
+                       stream, V, ldv, m, h, w, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpby(double alpha, const double *x, double beta, double *y,
+                        int64_t n, hipStream_t stream)
+{
+    hipLaunchKernelGGL(axpby_kernel, dim3(stream_grid(n)), dim3(256), 0,
+                       stream, alpha, x, beta, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t stream)
+{
+    hipLaunchKernelGGL(fill_kernel, dim3(stream_grid(n)), dim3(256), 0,
+                       stream, x, v, n);
+    return hipGetLastError();
+}
+
+template hipError_t launch_sweep_step<double>(const SweepArgs<double> &, int,
+                                              int, int, hipStream_t);
+template hipError_t launch_sweep_step<float>(const SweepArgs<float> &, int,
+                                             int, int, hipStream_t);
+template hipError_t launch_lag_gather<double>(const int64_t *, const int32_t *,
+                                              int64_t, int64_t, int,
+                                              const uint32_t *, const double *,
+                                              double *, hipStream_t);
+template hipError_t launch_lag_gather<float>(const int64_t *, const int32_t *,
+                                             int64_t, int64_t, int,
+                                             const uint32_t *, const float *,
+                                             float *, hipStream_t);
+template hipError_t launch_psi_to_phi<double>(const double *, const double *,
+                                              int64_t, int64_t, double *,
+                                              hipStream_t);
+template hipError_t launch_psi_to_phi<float>(const float *, const double *,
+                                             int64_t, int64_t, double *,
+                                             hipStream_t);
+template hipError_t launch_total_source<double>(const double *,
+                                                const double *,
+                                                const double *, int64_t, int,
+                                                double *, hipStream_t);
+template hipError_t launch_total_source<float>(const double *, const double *,
+                                               const double *, int64_t, int,
+                                               float *, hipStream_t);
+template hipError_t launch_ac_to_ca<double>(const double *, double *, int64_t,
+                                            int64_t, int64_t, hipStream_t);
+template hipError_t launch_ac_to_ca<float>(const float *, double *, int64_t,
+                                           int64_t, int64_t, hipStream_t);
+
+}  // namespace t2

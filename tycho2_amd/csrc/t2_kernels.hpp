@@ -1,0 +1,91 @@
+// Device kernels of the sweep engine and their launch wrappers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace t2 {
+
+// Per-cell geometry record on the device: 4 outward unit normals, 4 face
+// areas, the volume, padded to 160 bytes.
+constexpr int kGeoStride = 20;
+constexpr int kGeoNormal = 0;
+constexpr int kGeoArea = 12;
+constexpr int kGeoVolume = 16;
+
+// Largest block of the sweep kernels (their __launch_bounds__).
+constexpr int kMaxBlock = 256;
+
+template <typename T>
+struct SweepArgs {
+    const uint32_t *work;      // step slice of the work list
+    int64_t n_items;
+    int cell_bits;
+    int64_t nc;
+    const double *omega;       // [na][3]
+    const double *geo;         // [nc][kGeoStride]
+    const int4 *adj;           // [nc] neighbour per face, -1 on the boundary
+    const uint32_t *nbr;       // [nc] packed neighbour vertices
+    const T *sigma_t;          // [nc][G]
+    const T *src_iso;          // [nc][4][G] isotropic total source
+    const double *src_ang;     // [na][nc][4][G] angle-dependent part or null
+    T *psi;                    // [na][nc][4][G]
+    const int64_t *lag_key;    // sorted ((a*nc + c)*4 + k), or null
+    int64_t n_lag;
+    const T *lag_buf;          // [n_lag][4][G] upwind values held pre-sweep
+};
+
+// Sweep of one step: every work item's cell solve for all groups.
+template <typename T>
+hipError_t launch_sweep_step(const SweepArgs<T> &args, int groups,
+                             int groups_per_lane, int block,
+                             hipStream_t stream);
+
+// lag_buf[i][j][g] = psi[a_i][u_i][vertex of u_i facing c_i's vertex j][g]
+template <typename T>
+hipError_t launch_lag_gather(const int64_t *lag_key, const int32_t *lag_up,
+                             int64_t n_lag, int64_t nc, int groups,
+                             const uint32_t *nbr, const T *psi, T *lag_buf,
+                             hipStream_t stream);
+
+// phi[c][v][g] = sum_a w_a psi[a][c][v][g]  (FP64 accumulation)
+template <typename T>
+hipError_t launch_psi_to_phi(const T *psi, const double *wt, int64_t na,
+                             int64_t n_phi, double *phi, hipStream_t stream);
+
+// out[c][v][g] = (q ? q[c][v][g] : 0) + sigma_s[c][g] / (4 pi) * phi[c][v][g]
+template <typename T>
+hipError_t launch_total_source(const double *q, const double *sigma_s,
+                               const double *phi, int64_t nc, int groups,
+                               T *out, hipStream_t stream);
+
+// Layout changes between the reference's [c][a][v][g] and the engine's
+// [a][c][v][g] (vg = 4 * groups contiguous values per (cell, angle)).
+hipError_t launch_ca_to_ac(const double *in, double *out, int64_t nc,
+                           int64_t na, int64_t vg, hipStream_t stream);
+template <typename T>
+hipError_t launch_ac_to_ca(const T *in, double *out, int64_t nc, int64_t na,
+                           int64_t vg, hipStream_t stream);
+
+// Deterministic reductions: partial sums per block into `partial`
+// (kRedBlocks entries per result), then one block adds them in order.
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+// out[0] = sum |x - y|, out[1] = sum |x|
+hipError_t launch_l1_change(const double *x, const double *y, int64_t n,
+                            double *partial, double *out, hipStream_t stream);
+// out[j] = sum_i V[j][i] w[i], j < m  (V holds m vectors of length n)
+hipError_t launch_multi_dot(const double *V, int64_t ldv, int m,
+                            const double *w, int64_t n, double *partial,
+                            double *out, hipStream_t stream);
+// w -= sum_j h[j] V[j]  (h on the device)
+hipError_t launch_multi_axpy(const double *V, int64_t ldv, int m,
+                             const double *h, double *w, int64_t n,
+                             hipStream_t stream);
+// y = alpha * x + beta * y
+hipError_t launch_axpby(double alpha, const double *x, double beta, double *y,
+                        int64_t n, hipStream_t stream);
+hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t stream);
+
+}  // namespace t2

@@ -1,0 +1,275 @@
+"""Python front end of the MI355X sweep engine (``libtycho2_sweep.so``).
+
+``SweeperHIP`` mirrors the reference's sweeper plug-in interface
+(src/SweeperAbstract.hh:47-70): ``sweep(psi, source, zero_psi_bound)``
+overwrites ``psi`` with L^-1 ``source`` on host arrays in the reference
+layout [cell][angle][vertex][group], and ``solve()`` runs the outer
+iteration (source iteration or GMRES, src/SourceIteration.cc) on the device.
+Every call goes through the C ABI declared in include/tycho2_sweep.h; there
+is no fallback: a missing library or a failing call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import build as _build
+
+FP64 = 0
+FP32 = 1
+GAUSS_NOPIVOT = 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_LIB = None
+
+# Every exported symbol of include/tycho2_sweep.h with its argument types.
+SIGNATURES = {
+    "t2_last_error": (ctypes.c_char_p, []),
+    "t2_abi_version": (ctypes.c_int, []),
+    "t2_mesh_geometry": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, _dp,
+                                        _ip, _ip, _dp, _dp, _dp]),
+    "t2_sweep_levels": (ctypes.c_int, [ctypes.c_int64, _ip, _dp,
+                                       ctypes.c_int64, _dp, _i32p, _ip]),
+    "t2_sweeper_create": (ctypes.c_int, [
+        ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int64,
+        ctypes.c_int64, _dp, _ip, _ip, _ip, ctypes.c_int64, _dp, _dp,
+        ctypes.c_int64, _dp, _dp, ctypes.c_int, ctypes.c_int]),
+    "t2_sweeper_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "t2_sweeper_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "t2_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.c_int]),
+    "t2_sweep_host": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp,
+                                     ctypes.c_int]),
+    "t2_set_source_isotropic": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "t2_set_source_angular": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "t2_solve_source_iteration": (ctypes.c_int, [
+        ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, _ip, _dp]),
+    "t2_solve_krylov": (ctypes.c_int, [
+        ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int64,
+        _ip, _dp]),
+    "t2_sweep_step": (ctypes.c_int, [ctypes.c_void_p]),
+    "t2_get_phi": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "t2_get_psi": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "t2_set_phi": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "t2_get_psi_angle": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64,
+                                        _dp]),
+    "t2_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "t2_get_stream": (ctypes.c_int, [ctypes.c_void_p,
+                                     ctypes.POINTER(ctypes.c_void_p)]),
+}
+
+
+class T2Info(ctypes.Structure):
+    _fields_ = [("n_cells", ctypes.c_int64), ("n_angles", ctypes.c_int64),
+                ("n_groups", ctypes.c_int64), ("n_steps", ctypes.c_int64),
+                ("min_levels", ctypes.c_int64), ("n_lagged", ctypes.c_int64),
+                ("psi_bytes", ctypes.c_int64),
+                ("device_bytes", ctypes.c_int64),
+                ("precision", ctypes.c_int32), ("gauss", ctypes.c_int32)]
+
+
+class T2Error(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def lib():
+    """Load the engine library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        path = _build.LIBPATH
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing; build it with "
+                               "`python -c 'import __graft_entry__ as g; "
+                               "g.build()'`")
+        L = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def library_path() -> str:
+    return _build.LIBPATH
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().t2_last_error()
+        raise T2Error(rc, msg.decode() if msg else "")
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def _ptr(a, kind=_dp):
+    return a.ctypes.data_as(kind)
+
+
+def mesh_geometry(smesh, conn):
+    """(volume, area, normal) of every cell, computed by the engine's host
+    code exactly as the reference's TychoMesh constructor does."""
+    nc = smesh.n_cells
+    vol, area, nrm = np.empty(nc), np.empty((nc, 4)), np.empty((nc, 4, 3))
+    xyz, cn, f2c = _f64(smesh.coords), _i64(smesh.cell_nodes), _i64(conn.f2c)
+    _check(lib().t2_mesh_geometry(nc, smesh.n_nodes, _ptr(xyz),
+                                  _ptr(cn, _ip), _ptr(f2c, _ip), _ptr(vol),
+                                  _ptr(area), _ptr(nrm)))
+    return vol, area, nrm
+
+
+def sweep_levels(adj, normal, omega):
+    """(levels[a][c], n_lagged) of the engine's per-angle schedule."""
+    adj, nrm, om = _i64(adj), _f64(normal), _f64(omega)
+    nc, na = adj.shape[0], om.shape[0]
+    levels = np.empty((na, nc), dtype=np.int32)
+    lag = ctypes.c_int64(0)
+    _check(lib().t2_sweep_levels(nc, _ptr(adj, _ip), _ptr(nrm), na, _ptr(om),
+                                 _ptr(levels, _i32p), ctypes.byref(lag)))
+    return levels, lag.value
+
+
+class SweeperHIP:
+    """Device sweeper for one mesh, quadrature and cross-section set."""
+
+    def __init__(self, smesh, conn, omega, wt, ng, sigma_t, sigma_s,
+                 precision: int = FP64, device: int = 0):
+        nc = smesh.n_cells
+        self.nc, self.na, self.ng = nc, int(np.shape(omega)[0]), int(ng)
+        st = _f64(np.broadcast_to(sigma_t, (nc, ng)))
+        ss = _f64(np.broadcast_to(sigma_s, (nc, ng)))
+        xyz, cn = _f64(smesh.coords), _i64(smesh.cell_nodes)
+        adj, f2c = _i64(conn.adj_cell), _i64(conn.f2c)
+        om, w = _f64(omega), _f64(wt)
+        h = ctypes.c_void_p()
+        _check(lib().t2_sweeper_create(
+            ctypes.byref(h), device, nc, smesh.n_nodes, _ptr(xyz),
+            _ptr(cn, _ip), _ptr(adj, _ip), _ptr(f2c, _ip), self.na, _ptr(om),
+            _ptr(w), self.ng, _ptr(st), _ptr(ss), precision, GAUSS_NOPIVOT))
+        self._h = h
+
+    @classmethod
+    def from_problem(cls, p, precision: int = FP64, device: int = 0):
+        s = cls(p.smesh, p.conn, p.omega, p.wt, p.ng, p.sigma_t, p.sigma_s,
+                precision, device)
+        if p.q_iso is not None:
+            s.set_source_isotropic(p.q_iso)
+        return s
+
+    # -- lifetime
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().t2_sweeper_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise T2Error(-4, "sweeper is closed")
+        return self._h
+
+    def info(self) -> dict:
+        i = T2Info()
+        _check(lib().t2_sweeper_info(self.handle, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in T2Info._fields_}
+
+    def set_launch(self, items_per_block: int = 0, groups_per_lane: int = 0):
+        _check(lib().t2_set_launch(self.handle, items_per_block,
+                                   groups_per_lane))
+
+    # -- SweeperAbstract::sweep
+    def sweep(self, psi: np.ndarray, source: np.ndarray,
+              zero_psi_bound: bool = False) -> np.ndarray:
+        """psi[:] = L^-1 source (host arrays, [cell][angle][vertex][group])."""
+        shape = (self.nc, self.na, 4, self.ng)
+        if psi.shape != shape or psi.dtype != np.float64 or \
+                not psi.flags.c_contiguous:
+            raise ValueError(f"psi must be a C-contiguous float64 {shape}")
+        src = _f64(source).reshape(shape)
+        _check(lib().t2_sweep_host(self.handle, _ptr(psi), _ptr(src),
+                                   int(bool(zero_psi_bound))))
+        return psi
+
+    # -- fixed source and outer iterations
+    def set_source_isotropic(self, q: Optional[np.ndarray]) -> None:
+        if q is None:
+            _check(lib().t2_set_source_isotropic(self.handle, None))
+            return
+        qq = _f64(q).reshape(self.nc, 4, self.ng)
+        _check(lib().t2_set_source_isotropic(self.handle, _ptr(qq)))
+
+    def set_source_angular(self, source: np.ndarray) -> None:
+        src = _f64(source).reshape(self.nc, self.na, 4, self.ng)
+        _check(lib().t2_set_source_angular(self.handle, _ptr(src)))
+
+    def solve_source_iteration(self, iter_max=100, err_max=1e-10):
+        hist = np.zeros(max(int(iter_max), 1))
+        nit = ctypes.c_int64(0)
+        _check(lib().t2_solve_source_iteration(
+            self.handle, int(iter_max), float(err_max), ctypes.byref(nit),
+            _ptr(hist)))
+        return hist[:nit.value].copy()
+
+    def solve_krylov(self, iter_max=100, rtol=1e-10, restart=30):
+        hist = np.zeros(max(int(iter_max), 1))
+        nit = ctypes.c_int64(0)
+        _check(lib().t2_solve_krylov(self.handle, int(iter_max), float(rtol),
+                                     int(restart), ctypes.byref(nit),
+                                     _ptr(hist)))
+        return hist[:nit.value].copy()
+
+    def sweep_step(self) -> None:
+        _check(lib().t2_sweep_step(self.handle))
+
+    def synchronize(self) -> None:
+        _check(lib().t2_synchronize(self.handle))
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        _check(lib().t2_get_stream(self.handle, ctypes.byref(s)))
+        return s.value or 0
+
+    # -- results
+    def phi(self) -> np.ndarray:
+        out = np.empty((self.nc, 4, self.ng))
+        _check(lib().t2_get_phi(self.handle, _ptr(out)))
+        return out
+
+    def set_phi(self, phi: np.ndarray) -> None:
+        p = _f64(phi).reshape(self.nc, 4, self.ng)
+        _check(lib().t2_set_phi(self.handle, _ptr(p)))
+
+    def psi_angle(self, angle: int) -> np.ndarray:
+        out = np.empty((self.nc, 4, self.ng))
+        _check(lib().t2_get_psi_angle(self.handle, int(angle), _ptr(out)))
+        return out
+
+    def psi(self) -> np.ndarray:
+        out = np.empty((self.nc, self.na, 4, self.ng))
+        _check(lib().t2_get_psi(self.handle, _ptr(out)))
+        return out
