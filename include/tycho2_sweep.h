@@ -83,10 +83,13 @@ int t2_mesh_geometry(int64_t n_cells, int64_t n_nodes, const double *coords,
 
 /* Level of every (angle, cell) in the per-angle sweep order, levels[a][c]
  * (replaces GraphTraverser's ready queue, src/GraphTraverser.cc:467-644).
- * Returns the number of lagged faces in *n_lagged (cycles, see DESIGN). */
+ * Dependency cycles, on which the reference's traversal never ends, are
+ * broken by lagging the weakest unmet face (smallest A |Omega.n|); the number
+ * of lagged (angle, cell, face) triples is returned in *n_lagged. */
 int t2_sweep_levels(int64_t n_cells, const int64_t *adj_cell,
-                    const double *normal, int64_t n_angles,
-                    const double *omega, int32_t *levels, int64_t *n_lagged);
+                    const double *area, const double *normal,
+                    int64_t n_angles, const double *omega, int32_t *levels,
+                    int64_t *n_lagged);
 
 /* ------------------------------------------------------------ setup
  * Replaces the reference's per-mesh set-up (TychoMesh, Quadrature,
@@ -141,6 +144,10 @@ int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,
 /* One sweep with the current scalar flux: psi = L^-1 (q + S phi) and
  * phi <- D psi (one source-iteration step; the benchmark's unit of work). */
 int t2_sweep_step(t2_sweeper *s);
+
+/* One sweep psi = L^-1 (current total source) without the source update or
+ * the moment (times the sweep kernels alone). */
+int t2_sweep_only(t2_sweeper *s);
 
 /* Copy results to the host in the reference layouts. */
 int t2_get_phi(const t2_sweeper *s, double *phi);

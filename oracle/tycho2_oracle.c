@@ -326,13 +326,23 @@ static inline double omega_dot_n(const double *om, const double *n)
     return om[0] * n[0] + om[1] * n[1] + om[2] * n[2];
 }
 
+/* Lagged faces of one angle: (cell, face) and the upwind values the face
+ * reads, psi[upwind][.][a][.] as held before the sweep. */
+typedef struct {
+    int64_t cell;
+    int face;
+    double *vals; /* [4][ng], entry j = k unused */
+} or_lag;
+
 /* Update of one (cell, angle): SweepData::update (SweepData.hh:139-174) with
  * the upwind gather of Transport::populateLocalPsiBound (Transport.cc:740-776)
  * folded in.  Face k's value at cell vertex j is the neighbour's psi at
- * nbrv[c][k][c2f[c][k][j]]; exterior faces give zero (vacuum). */
+ * nbrv[c][k][c2f[c][k][j]]; exterior faces give zero (vacuum); lagged faces
+ * read their snapshot. */
 static void update_cell(const or_mesh *m, int64_t c, int64_t a, int64_t na,
                         const double *om, int64_t ng, const double *sigma_t,
-                        const double *source, double *psi, int method)
+                        const double *source, double *psi, int method,
+                        const or_lag *lag, int64_t n_lag)
 {
     double abar[4], odn[4];
     for (int k = 0; k < 4; k++) {
@@ -350,9 +360,17 @@ static void update_cell(const or_mesh *m, int64_t c, int64_t a, int64_t na,
             int64_t nb = m->adj[4 * c + k];
             if (odn[k] > 0 || nb < 0)
                 continue;
+            const or_lag *l = NULL;
+            for (int64_t i = 0; i < n_lag; i++)
+                if (lag[i].cell == c && lag[i].face == k)
+                    l = lag + i;
             for (int j = 0; j < 4; j++) {
                 if (j == k)
                     continue;
+                if (l) {
+                    up[k][j] = l->vals[j * ng + g];
+                    continue;
+                }
                 int64_t fv = m->c2f[(4 * c + k) * 4 + j];
                 int64_t nv = m->nbrv[(4 * c + k) * 3 + fv];
                 up[k][j] = psi[((nb * na + a) * 4 + nv) * ng + g];
@@ -371,27 +389,38 @@ static void update_cell(const or_mesh *m, int64_t c, int64_t a, int64_t na,
  * cells gives the same psi.
  *
  * Cycles (the reference loops forever on them, GraphTraverser.cc:518): when
- * no cell is ready, the unfinished cell with the fewest unmet upwind faces
- * (lowest index on ties) is released; its unmet faces are "lagged" and read
- * the psi the array held before this sweep, as the reference's PBJ sweeps
- * read last iterate's partition faces (SweeperPBJ.cc:168-219).  The set of
- * cells finished when the traversal stalls does not depend on the order, so
- * the lagged set is schedule independent.
+ * no cell is ready, the weakest unmet face -- smallest A |Omega.n| over every
+ * interior incoming face of an unfinished cell whose upwind cell is
+ * unfinished, lowest (cell, face) on ties -- is lagged, and the traversal
+ * goes on.  A lagged face reads the upwind psi held before this sweep, as the
+ * reference's PBJ sweeps read last iterate's partition faces
+ * (SweeperPBJ.cc:168-219); at a converged iteration that is the current psi.
+ * The set of finished cells at a stall does not depend on the traversal
+ * order, so neither does the lagged set.
+ *
+ * Pass 1 fixes the order and the lagged faces, the snapshot of the lagged
+ * faces' upwind values is taken, and pass 2 computes in that order.
  */
 static int64_t sweep_angle(const or_mesh *m, int64_t a, int64_t na,
                            const double *om, int64_t ng,
                            const double *sigma_t, const double *source,
                            double *psi, int method, int64_t *indeg,
-                           int64_t *queue, unsigned char *queued)
+                           int64_t *queue, unsigned char *queued,
+                           unsigned char *met)
 {
     const int64_t nc = m->nc;
-    int64_t head = 0, tail = 0, lagged = 0;
+    int64_t head = 0, tail = 0, n_lag = 0, cap = 0;
+    or_lag *lag = NULL;
+    /* met[c][k]: 0 unmet incoming interior face, 1 released by the upwind
+     * cell, 2 lagged, 3 not a dependency */
     for (int64_t c = 0; c < nc; c++) {
         int64_t d = 0;
-        for (int k = 0; k < 4; k++)
-            if (m->adj[4 * c + k] >= 0 &&
-                !(omega_dot_n(om, m->normal + (4 * c + k) * 3) > 0))
-                d++;
+        for (int k = 0; k < 4; k++) {
+            int dep = m->adj[4 * c + k] >= 0 &&
+                      !(omega_dot_n(om, m->normal + (4 * c + k) * 3) > 0);
+            met[4 * c + k] = dep ? 0 : 3;
+            d += dep;
+        }
         indeg[c] = d;
         queued[c] = 0;
     }
@@ -402,30 +431,82 @@ static int64_t sweep_angle(const or_mesh *m, int64_t a, int64_t na,
         }
     while (head < nc) {
         if (head == tail) {
-            int64_t best = -1;
-            for (int64_t c = 0; c < nc; c++)
-                if (!queued[c] && (best < 0 || indeg[c] < indeg[best]))
-                    best = c;
-            lagged += indeg[best];
-            indeg[best] = 0;
-            queue[tail++] = best;
-            queued[best] = 1;
+            int64_t bc = -1;
+            int bk = -1;
+            double bw = 0.0;
+            for (int64_t c = 0; c < nc; c++) {
+                if (queued[c])
+                    continue;
+                for (int k = 0; k < 4; k++) {
+                    if (met[4 * c + k] != 0)
+                        continue;
+                    double d = omega_dot_n(om, m->normal + (4 * c + k) * 3);
+                    double wgt = m->area[4 * c + k] * fabs(d);
+                    if (bc < 0 || wgt < bw) {
+                        bc = c;
+                        bk = k;
+                        bw = wgt;
+                    }
+                }
+            }
+            if (bc < 0)
+                abort(); /* unreachable: a stalled cell has an unmet face */
+            if (n_lag == cap) {
+                cap = cap ? 2 * cap : 16;
+                lag = realloc(lag, sizeof(or_lag) * cap);
+            }
+            lag[n_lag].cell = bc;
+            lag[n_lag].face = bk;
+            lag[n_lag].vals = NULL;
+            n_lag++;
+            met[4 * bc + bk] = 2;
+            if (--indeg[bc] == 0) {
+                queue[tail++] = bc;
+                queued[bc] = 1;
+            }
+            continue;
         }
         int64_t c = queue[head++];
-        update_cell(m, c, a, na, om, ng, sigma_t, source, psi, method);
         for (int k = 0; k < 4; k++) {
             int64_t nb = m->adj[4 * c + k];
-            if (nb < 0 || queued[nb])
+            if (nb < 0 || queued[nb] ||
+                !(omega_dot_n(om, m->normal + (4 * c + k) * 3) > 0))
                 continue;
-            if (omega_dot_n(om, m->normal + (4 * c + k) * 3) > 0) {
-                if (--indeg[nb] == 0) {
-                    queue[tail++] = nb;
-                    queued[nb] = 1;
-                }
+            int back = 0; /* nb's face towards c */
+            while (m->adj[4 * nb + back] != c)
+                back++;
+            if (met[4 * nb + back] != 0)
+                continue;
+            met[4 * nb + back] = 1;
+            if (--indeg[nb] == 0) {
+                queue[tail++] = nb;
+                queued[nb] = 1;
             }
         }
     }
-    return lagged;
+    /* snapshot of the lagged faces' upwind values */
+    for (int64_t i = 0; i < n_lag; i++) {
+        const int64_t c = lag[i].cell;
+        const int k = lag[i].face;
+        const int64_t nb = m->adj[4 * c + k];
+        lag[i].vals = calloc(4 * ng, sizeof(double));
+        for (int j = 0; j < 4; j++) {
+            if (j == k)
+                continue;
+            int64_t fv = m->c2f[(4 * c + k) * 4 + j];
+            int64_t nv = m->nbrv[(4 * c + k) * 3 + fv];
+            for (int64_t g = 0; g < ng; g++)
+                lag[i].vals[j * ng + g] =
+                    psi[((nb * na + a) * 4 + nv) * ng + g];
+        }
+    }
+    for (int64_t i = 0; i < nc; i++)
+        update_cell(m, queue[i], a, na, om, ng, sigma_t, source, psi, method,
+                    lag, n_lag);
+    for (int64_t i = 0; i < n_lag; i++)
+        free(lag[i].vals);
+    free(lag);
+    return n_lag;
 }
 
 int or_sweep_angles(const or_mesh *m, int64_t na, const double *omega,
@@ -445,7 +526,8 @@ int or_sweep_angles(const or_mesh *m, int64_t na, const double *omega,
         int64_t *indeg = malloc(sizeof(int64_t) * nc);
         int64_t *queue = malloc(sizeof(int64_t) * nc);
         unsigned char *queued = malloc(nc);
-        if (!indeg || !queue || !queued) {
+        unsigned char *lagged = malloc(4 * nc);
+        if (!indeg || !queue || !queued || !lagged) {
 #ifdef _OPENMP
 #pragma omp atomic write
 #endif
@@ -457,11 +539,12 @@ int or_sweep_angles(const or_mesh *m, int64_t na, const double *omega,
             for (int64_t a = a0; a < a1; a++)
                 lag_total += sweep_angle(m, a, na, omega + 3 * a, ng, sigma_t,
                                          source, psi, method, indeg, queue,
-                                         queued);
+                                         queued, lagged);
         }
         free(indeg);
         free(queue);
         free(queued);
+        free(lagged);
     }
     (void)nthreads;
     if (n_lagged)

@@ -1,0 +1,253 @@
+"""Pins the CPU oracle against mathematics it must satisfy: an independent
+quadrature assembly of the DG system, closed-form solutions, quadrature
+moment identities, Kuhn-mesh geometry and dense linear algebra."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from tycho2_amd import mesh as meshmod
+from tycho2_amd.quadrature import product_quadrature
+
+from _dg import face_data, global_sweep_solution, local_system
+
+
+def random_tet(rng):
+    while True:
+        P = rng.uniform(-1, 1, (4, 3))
+        if abs(np.linalg.det(np.hstack([P, np.ones((4, 1))]))) > 0.3:
+            return P
+
+
+def oracle_local(P, omega, sigma, q, upwind, method=orc.NOPIVOT):
+    """Run the oracle's cell solve on the geometry of tet P."""
+    n, area = face_data(P)
+    vol = abs(np.linalg.det(np.hstack([P, np.ones((4, 1))]))) / 6.0
+    abar = area * (n @ omega)
+    up = np.zeros((4, 4))
+    for k in range(4):
+        if upwind[k] is not None:
+            up[k] = upwind[k]
+    return orc.cell_solve(vol, abar, sigma, q, up, method)
+
+
+@pytest.mark.parametrize("seed", range(20))
+@pytest.mark.parametrize("method", [orc.NOPIVOT, orc.ORIGINAL])
+def test_cell_solve_matches_weak_form(seed, method):
+    rng = np.random.default_rng(seed)
+    P = random_tet(rng)
+    omega = rng.normal(size=3)
+    omega /= np.linalg.norm(omega)
+    sigma = rng.uniform(0.1, 5.0)
+    q = rng.uniform(-1, 2, 4)
+    upwind = [rng.uniform(-1, 1, 4) for _ in range(4)]
+    M, rhs = local_system(P, omega, sigma, q, upwind)
+    want = np.linalg.solve(M, rhs)
+    got = oracle_local(P, omega, sigma, q, upwind, method)
+    assert np.allclose(got, want, rtol=1e-11, atol=1e-12)
+
+
+def test_constant_preserved():
+    """psi = c with inflow c and q = sigma c is reproduced exactly (the
+    streaming of a constant integrates to zero)."""
+    rng = np.random.default_rng(1)
+    for _ in range(10):
+        P = random_tet(rng)
+        omega = rng.normal(size=3)
+        omega /= np.linalg.norm(omega)
+        sigma, c = rng.uniform(0.2, 3.0), rng.uniform(0.5, 2.0)
+        got = oracle_local(P, omega, sigma, np.full(4, sigma * c),
+                           [np.full(4, c)] * 4)
+        assert np.allclose(got, c, rtol=1e-13)
+
+
+def test_linear_solution_exact():
+    """A linear psi with q = Omega.grad psi + sigma psi and exact inflow
+    traces is reproduced at the vertices."""
+    rng = np.random.default_rng(2)
+    for _ in range(10):
+        P = random_tet(rng)
+        omega = rng.normal(size=3)
+        omega /= np.linalg.norm(omega)
+        sigma = rng.uniform(0.2, 3.0)
+        alpha, beta = rng.normal(), rng.normal(size=3)
+        exact = alpha + P @ beta
+        q = omega @ beta + sigma * exact
+        got = oracle_local(P, omega, sigma, q, [exact] * 4)
+        assert np.allclose(got, exact, rtol=1e-12, atol=1e-12)
+
+
+def test_pivot_and_nopivot_agree():
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        P = random_tet(rng)
+        omega = rng.normal(size=3)
+        omega /= np.linalg.norm(omega)
+        args = (P, omega, rng.uniform(0.1, 10), rng.uniform(0, 1, 4),
+                [rng.uniform(0, 1, 4) for _ in range(4)])
+        a = oracle_local(*args, method=orc.NOPIVOT)
+        b = oracle_local(*args, method=orc.ORIGINAL)
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("sn", [2, 4, 8, 12, 16])
+def test_quadrature_moments(sn):
+    om, w = orc.quadrature(sn)
+    assert om.shape == (2 * sn * sn, 3)
+    assert np.allclose(np.linalg.norm(om, axis=1), 1.0, atol=1e-15)
+    assert abs(w.sum() - 4 * np.pi) < 1e-13
+    assert np.allclose(w @ om, 0.0, atol=1e-13)
+    second = (om * w[:, None]).T @ om
+    assert np.allclose(second, 4 * np.pi / 3 * np.eye(3), atol=1e-13)
+
+
+def test_quadrature_matches_engine_tables():
+    """Oracle and engine compute the same Gauss-Legendre tables
+    independently (decimal Newton vs long-double Newton); directions agree to
+    an ulp (the libm sincos/cos paths may differ), weights exactly."""
+    for sn in range(2, 31, 2):
+        a, wa = orc.quadrature(sn)
+        b, wb = product_quadrature(sn)
+        assert np.array_equal(wa, wb)
+        assert np.abs(a - b).max() <= 2.3e-16
+
+
+def test_kuhn_geometry_closed_forms():
+    n = 4
+    sm = meshmod.kuhn_cube(n)
+    m = orc.build_mesh(sm)
+    h = 1.0 / n
+    assert np.allclose(m.volume, h ** 3 / 6, rtol=1e-14)
+    # divergence theorem: sum_k A_k n_k = 0 for every tet
+    s = (m.area[:, :, None] * m.normal).sum(axis=1)
+    assert np.abs(s).max() < 1e-15
+    # neighbours see the opposite normal, bit for bit
+    for c in range(m.nc):
+        for k in range(4):
+            u = m.adj[c, k]
+            if u < 0:
+                continue
+            kk = int(np.nonzero(m.adj[u] == c)[0][0])
+            assert np.array_equal(m.normal[u, kk], -m.normal[c, k])
+    # mean upwind interior faces per cell = 2 - 1/n
+    interior = (m.adj >= 0).sum()
+    assert interior / 2 / m.nc == pytest.approx(2 - 1 / n, abs=1e-15)
+
+
+def test_jittered_geometry_positive():
+    sm = meshmod.kuhn_cube(6, 0.15, seed=3)
+    m = orc.build_mesh(sm)
+    assert (m.volume > 0).all()
+    s = (m.area[:, :, None] * m.normal).sum(axis=1)
+    assert np.abs(s).max() < 1e-14
+
+
+@pytest.mark.parametrize("which", ["cube12", "kuhn3"])
+def test_sweep_matches_dense_global_solve(which, cube12):
+    """One direction: the oracle's traversal equals the dense solve of the
+    global DG system assembled independently."""
+    if which == "cube12":
+        sm = cube12
+    else:
+        sm = meshmod.kuhn_cube(3, 0.15, seed=4)
+    m = orc.build_mesh(sm)
+    rng = np.random.default_rng(5)
+    omega = np.array([[0.3, -0.5, 0.8]])
+    omega /= np.linalg.norm(omega)
+    sig = rng.uniform(0.5, 2.0, m.nc)
+    q = rng.uniform(0.0, 1.0, (m.nc, 4))
+    want = global_sweep_solution(sm.coords, sm.cell_nodes, m.adj, omega[0],
+                                 sig, q)
+    psi, lag = orc.sweep(m, omega, 1, sig, q[:, None, :, None])
+    assert lag == 0
+    assert np.allclose(psi[:, 0, :, 0], want, rtol=1e-11,
+                       atol=1e-12 * np.abs(want).max())
+
+
+def test_psi_to_phi_closed_form():
+    om, w = orc.quadrature(6)
+    psi = np.ones((5, om.shape[0], 4, 3))
+    phi = orc.psi_to_phi(psi, w)
+    assert np.allclose(phi, 4 * np.pi, rtol=1e-14)
+
+
+def test_gmres_dense_textbook():
+    rng = np.random.default_rng(7)
+    n = 60
+    A = np.eye(n) + 0.3 * rng.normal(size=(n, n)) / np.sqrt(n)
+    b = rng.normal(size=n)
+    x, it = orc.gmres_dense(A, b, restart=10, iter_max=500, rtol=1e-13)
+    assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+    assert it < 500
+
+
+def _small_transport(n=2, sn=4, ng=2, seed=0):
+    sm = meshmod.kuhn_cube(n, 0.1, seed=seed)
+    m = orc.build_mesh(sm)
+    om, w = orc.quadrature(sn)
+    rng = np.random.default_rng(seed)
+    st = rng.uniform(1.0, 2.0, (m.nc, ng))
+    ss = st * 0.8
+    src = orc.isotropic_source(m.nc, om.shape[0], ng, np.ones((m.nc, ng)))
+    return m, om, w, ng, st, ss, src
+
+
+def test_source_iteration_converges_to_dense_fixed_point():
+    """SI's converged phi solves (I - D L^-1 M S) phi = D L^-1 q, the
+    operator built column by column from single sweeps."""
+    m, om, w, ng, st, ss, src = _small_transport()
+    nphi = m.nc * 4 * ng
+    na = om.shape[0]
+    assert orc.sweep(m, om, ng, st, src)[1] == 0   # no lagged faces
+
+    def DLinv(source):
+        psi, _ = orc.sweep(m, om, ng, st, source)
+        return orc.psi_to_phi(psi, w).ravel()
+
+    b = DLinv(src)
+    A = np.zeros((nphi, nphi))
+    for i in range(nphi):
+        e = np.zeros(nphi)
+        e[i] = 1.0
+        ephi = e.reshape(m.nc, 4, ng)
+        tot = (ss / (4 * np.pi))[:, None, None, :] * \
+            np.broadcast_to(ephi[:, None], (m.nc, na, 4, ng))
+        A[:, i] = e - DLinv(np.ascontiguousarray(tot))
+    want = np.linalg.solve(A, b)
+    _, phi, hist = orc.source_iteration(m, om, w, ng, st, ss, src, 5000,
+                                        1e-15)
+    assert np.allclose(phi.ravel(), want, rtol=1e-11)
+    assert hist[-1] <= 1e-15 or len(hist) == 5000
+
+
+def test_krylov_equals_source_iteration():
+    m, om, w, ng, st, ss, src = _small_transport(n=3, sn=4)
+    _, phi_si, _ = orc.source_iteration(m, om, w, ng, st, ss, src, 5000,
+                                        1e-15)
+    _, phi_k, hist = orc.krylov(m, om, w, ng, st, ss, src, 200, 1e-13, 30)
+    assert len(hist) < 200
+    assert np.allclose(phi_k, phi_si, rtol=1e-11)
+
+
+def test_cyclic_mesh_source_iteration_converges_to_coupled_solution():
+    """On a mesh whose sweep graph has cycles (S2 directions make the Kuhn
+    diagonal faces nearly parallel to Omega) the reference never finishes a
+    sweep.  With lagged faces reading the previous iterate, source iteration
+    converges to the solution of the fully coupled DG system of every
+    direction (dense global solve)."""
+    sm = meshmod.kuhn_cube(3, 0.2, seed=3)
+    m = orc.build_mesh(sm)
+    om, w = orc.quadrature(2)
+    ng = 1
+    rng = np.random.default_rng(8)
+    st = rng.uniform(1.0, 2.0, (m.nc, ng))
+    ss = 0.5 * st
+    src = orc.isotropic_source(m.nc, om.shape[0], ng, np.ones((m.nc, ng)))
+    assert orc.sweep(m, om, ng, st, src)[1] > 0
+    psi, phi, hist = orc.source_iteration(m, om, w, ng, st, ss, src, 2000,
+                                          1e-15)
+    tot = src + (ss / (4 * np.pi))[:, None, None, :] * phi[:, None]
+    for a in range(om.shape[0]):
+        want = global_sweep_solution(sm.coords, sm.cell_nodes, m.adj, om[a],
+                                     st[:, 0], tot[:, a, :, 0])
+        assert np.allclose(psi[:, a, :, 0], want, rtol=1e-12,
+                           atol=1e-13 * np.abs(want).max())

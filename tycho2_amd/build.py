@@ -18,12 +18,14 @@ LIBNAME = "libtycho2_sweep.so"
 LIBPATH = os.path.join(HERE, LIBNAME)
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
+             "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+             f"-I{INCLUDE}"]
 HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall",
-              "-Wno-unused-function"]
+              "-Wno-unused-function", f"-I{INCLUDE}"]
 
 
 def _hipcc() -> str:

@@ -43,11 +43,13 @@ struct t2_sweeper {
     size_t bytes = 0;
     std::vector<void *> allocs;
 
-    double *d_omega = nullptr, *d_wt = nullptr, *d_geo = nullptr;
+    double *d_omega = nullptr, *d_wt = nullptr;
     double *d_sigs = nullptr;
-    void *d_sigt = nullptr;          // T [nc][G]
-    int4 *d_adj = nullptr;
-    uint32_t *d_nbr = nullptr, *d_work = nullptr;
+    int sigma_mode = t2::kSigmaCellGroup;
+    void *d_sigt = nullptr;          // T [nc][G] or [G] (by sigma_mode)
+    uint32_t *d_nbr = nullptr;
+    void *d_item_geo = nullptr;      // ItemGeo<T> [nc * na], step order
+    t2::ItemTopo *d_item_topo = nullptr;
     void *d_psi = nullptr;           // T [na][nc][4][G]
     void *d_src = nullptr;           // T [nc][4][G]
     double *d_qiso = nullptr;        // FP64 [nc][4][G]
@@ -130,18 +132,12 @@ template <typename T>
 t2::SweepArgs<T> base_args(const t2_sweeper *s, const double *ang)
 {
     t2::SweepArgs<T> a{};
-    a.cell_bits = s->sched.cell_bits;
     a.nc = s->nc;
-    a.omega = s->d_omega;
-    a.geo = s->d_geo;
-    a.adj = s->d_adj;
-    a.nbr = s->d_nbr;
-    a.sigma_t = static_cast<const T *>(s->d_sigt);
+    a.sigma_mode = s->sigma_mode;
+    a.sigma = static_cast<const T *>(s->d_sigt);
     a.src_iso = static_cast<const T *>(s->d_src);
     a.src_ang = ang;
     a.psi = static_cast<T *>(s->d_psi);
-    a.lag_key = s->d_lag_key;
-    a.n_lag = int64_t(s->sched.lagged.size());
     a.lag_buf = static_cast<const T *>(s->d_lag_buf);
     return a;
 }
@@ -157,9 +153,11 @@ int enqueue_sweep(t2_sweeper *s, const double *ang)
             static_cast<const T *>(s->d_psi), static_cast<T *>(s->d_lag_buf),
             s->stream));
     t2::SweepArgs<T> a = base_args<T>(s, ang);
+    const auto *geo = static_cast<const t2::ItemGeo<T> *>(s->d_item_geo);
     for (int64_t st = 0; st < s->sched.n_steps; st++) {
         const int64_t off = s->sched.step_off[st];
-        a.work = s->d_work + off;
+        a.geo = geo + off;
+        a.topo = s->d_item_topo + off;
         a.n_items = s->sched.step_off[st + 1] - off;
         T2_HIP(t2::launch_sweep_step<T>(a, int(s->ng), s->gpl, s->block,
                                         s->stream));
@@ -264,6 +262,158 @@ int check(const t2_sweeper *s)
     return set_device(s);
 }
 
+template <typename P>
+int upload(P *dst, const P *src, size_t count)
+{
+    T2_HIP(hipMemcpy(dst, src, sizeof(P) * count, hipMemcpyHostToDevice));
+    return T2_OK;
+}
+
+// sigma_t layout on the device: per group when every cell agrees, per cell
+// (inside the item records) when every group agrees, else [cell][group].
+int pick_sigma_mode(int64_t nc, int64_t ng, const double *sig)
+{
+    bool by_group = true, by_cell = true;
+    for (int64_t c = 0; c < nc && (by_group || by_cell); c++)
+        for (int64_t g = 0; g < ng; g++) {
+            by_group &= sig[c * ng + g] == sig[g];
+            by_cell &= sig[c * ng + g] == sig[c * ng];
+        }
+    if (by_cell)
+        return t2::kSigmaCell;
+    return by_group ? t2::kSigmaGroup : t2::kSigmaCellGroup;
+}
+
+template <typename T>
+int build_items(t2_sweeper *s, const double *d_geo, const int4 *d_adj,
+                const uint32_t *d_work, const double *d_sig_cell)
+{
+    const int64_t n = s->nc * s->na;
+    int *d_err = nullptr;
+    T2_TRY(dalloc_t(s, &d_err, sizeof(int)));
+    T2_HIP(hipMemset(d_err, 0, sizeof(int)));
+    T2_HIP(t2::launch_build_items<T>(
+        d_work, n, s->sched.cell_bits, s->nc, s->d_omega, d_geo, d_adj,
+        s->d_nbr, d_sig_cell, s->d_lag_key, int64_t(s->sched.lagged.size()),
+        static_cast<t2::ItemGeo<T> *>(s->d_item_geo), s->d_item_topo, d_err,
+        s->stream));
+    int err = 0;
+    T2_HIP(hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost,
+                          s->stream));
+    T2_HIP(hipStreamSynchronize(s->stream));
+    dfree(s, d_err, sizeof(int));
+    if (err)
+        return fail(T2_ERR_ARG, "a cell has four incoming faces");
+    return T2_OK;
+}
+
+// Device set-up of a new handle: cross sections, item records in step
+// order, and the flux arrays.
+int setup_device(t2_sweeper *s, const t2::HostMesh &hm, const double *omega,
+                 const double *weights, const double *sigma_t,
+                 const double *sigma_s)
+{
+    const int64_t nc = s->nc, na = s->na, ng = s->ng;
+    const size_t ts = s->tsize();
+    const int64_t n_phi = nc * 4 * ng;
+    if (n_phi >= (int64_t(1) << 31))
+        return fail(T2_ERR_ARG, "cells x 4 x groups must stay below 2^31");
+    T2_TRY(dalloc_t(s, &s->d_omega, sizeof(double) * 3 * na));
+    T2_TRY(dalloc_t(s, &s->d_wt, sizeof(double) * na));
+    T2_TRY(dalloc_t(s, &s->d_sigs, sizeof(double) * nc * ng));
+    T2_TRY(upload(s->d_omega, omega, 3 * na));
+    T2_TRY(upload(s->d_wt, weights, na));
+    T2_TRY(upload(s->d_sigs, sigma_s, nc * ng));
+
+    s->sigma_mode = pick_sigma_mode(nc, ng, sigma_t);
+    const int64_t n_sig = s->sigma_mode == t2::kSigmaGroup       ? ng
+                          : s->sigma_mode == t2::kSigmaCellGroup ? nc * ng
+                                                                 : 1;
+    std::vector<char> sig(size_t(n_sig) * ts);
+    for (int64_t i = 0; i < n_sig; i++) {
+        const double v = s->sigma_mode == t2::kSigmaCell ? 0.0 : sigma_t[i];
+        if (s->precision == T2_FP64)
+            reinterpret_cast<double *>(sig.data())[i] = v;
+        else
+            reinterpret_cast<float *>(sig.data())[i] = float(v);
+    }
+    T2_TRY(dalloc(s, &s->d_sigt, sig.size()));
+    T2_HIP(hipMemcpy(s->d_sigt, sig.data(), sig.size(),
+                     hipMemcpyHostToDevice));
+
+    // set-up only: per-cell geometry, adjacency, work list
+    std::vector<double> geo(size_t(nc) * t2::kGeoStride, 0.0);
+    std::vector<int4> adj(nc);
+    std::vector<double> sig_cell(nc);
+    for (int64_t c = 0; c < nc; c++) {
+        double *g = &geo[size_t(c) * t2::kGeoStride];
+        for (int i = 0; i < 12; i++)
+            g[t2::kGeoNormal + i] = hm.normal[12 * c + i];
+        for (int k = 0; k < 4; k++)
+            g[t2::kGeoArea + k] = hm.area[4 * c + k];
+        g[t2::kGeoVolume] = hm.volume[c];
+        adj[c] = make_int4(int(hm.adj[4 * c]), int(hm.adj[4 * c + 1]),
+                           int(hm.adj[4 * c + 2]), int(hm.adj[4 * c + 3]));
+        sig_cell[c] = sigma_t[c * ng];
+    }
+    double *d_geo = nullptr, *d_sig_cell = nullptr;
+    int4 *d_adj = nullptr;
+    uint32_t *d_work = nullptr;
+    const size_t n_items = size_t(nc) * na;
+    T2_TRY(dalloc_t(s, &s->d_nbr, sizeof(uint32_t) * nc));
+    T2_TRY(upload(s->d_nbr, hm.nbr_pack.data(), nc));
+    T2_TRY(dalloc_t(s, &d_geo, sizeof(double) * geo.size()));
+    T2_TRY(dalloc_t(s, &d_adj, sizeof(int4) * nc));
+    T2_TRY(dalloc_t(s, &d_work, sizeof(uint32_t) * n_items));
+    T2_TRY(dalloc_t(s, &d_sig_cell, sizeof(double) * nc));
+    T2_TRY(upload(d_geo, geo.data(), geo.size()));
+    T2_TRY(upload(d_adj, adj.data(), nc));
+    T2_TRY(upload(d_work, s->sched.work.data(), n_items));
+    T2_TRY(upload(d_sig_cell, sig_cell.data(), nc));
+    std::vector<uint32_t>().swap(s->sched.work);
+
+    const int64_t n_lag = int64_t(s->sched.lagged.size());
+    if (n_lag > 0) {
+        std::vector<int64_t> key(n_lag);
+        std::vector<int32_t> up(n_lag);
+        for (int64_t i = 0; i < n_lag; i++) {
+            const auto &l = s->sched.lagged[i];
+            key[i] = (int64_t(l.angle) * nc + l.cell) * 4 + l.face;
+            up[i] = l.upwind;
+        }
+        T2_TRY(dalloc_t(s, &s->d_lag_key, sizeof(int64_t) * n_lag));
+        T2_TRY(dalloc_t(s, &s->d_lag_up, sizeof(int32_t) * n_lag));
+        T2_TRY(dalloc(s, &s->d_lag_buf, ts * size_t(n_lag) * 4 * ng));
+        T2_TRY(upload(s->d_lag_key, key.data(), n_lag));
+        T2_TRY(upload(s->d_lag_up, up.data(), n_lag));
+    }
+
+    const size_t geo_rec = s->precision == T2_FP64
+                               ? sizeof(t2::ItemGeo<double>)
+                               : sizeof(t2::ItemGeo<float>);
+    T2_TRY(dalloc(s, &s->d_item_geo, geo_rec * n_items));
+    T2_TRY(dalloc_t(s, &s->d_item_topo, sizeof(t2::ItemTopo) * n_items));
+    T2_TRY(s->precision == T2_FP64
+               ? build_items<double>(s, d_geo, d_adj, d_work, d_sig_cell)
+               : build_items<float>(s, d_geo, d_adj, d_work, d_sig_cell));
+    dfree(s, d_geo, sizeof(double) * geo.size());
+    dfree(s, d_adj, sizeof(int4) * nc);
+    dfree(s, d_work, sizeof(uint32_t) * n_items);
+    dfree(s, d_sig_cell, sizeof(double) * nc);
+
+    T2_TRY(dalloc(s, &s->d_psi, ts * size_t(na) * n_phi));
+    T2_TRY(dalloc(s, &s->d_src, ts * size_t(n_phi)));
+    T2_TRY(dalloc_t(s, &s->d_phi, sizeof(double) * n_phi));
+    T2_TRY(dalloc_t(s, &s->d_phi_old, sizeof(double) * n_phi));
+    T2_TRY(dalloc_t(s, &s->d_red, sizeof(double) * t2::kRedBlocks * 64));
+    T2_TRY(dalloc_t(s, &s->d_scal, sizeof(double) * 64));
+    T2_TRY(zero_psi(s));
+    T2_TRY(total_source(s, nullptr, nullptr));
+    T2_HIP(t2::launch_fill(s->d_phi, 0.0, n_phi, s->stream));
+    T2_HIP(hipStreamSynchronize(s->stream));
+    return T2_OK;
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -327,90 +477,9 @@ int t2_sweeper_create(t2_sweeper **out, int device, int64_t n_cells,
         delete s;
         return fail(T2_ERR_HIP, "stream creation failed");
     }
-    const int64_t nc = n_cells, na = n_angles, ng = n_groups;
-    const size_t ts = s->tsize();
-
-    // geometry records
-    std::vector<double> geo(size_t(nc) * t2::kGeoStride, 0.0);
-    std::vector<int4> adj(nc);
-    for (int64_t c = 0; c < nc; c++) {
-        double *g = &geo[size_t(c) * t2::kGeoStride];
-        for (int i = 0; i < 12; i++)
-            g[t2::kGeoNormal + i] = hm.normal[12 * c + i];
-        for (int k = 0; k < 4; k++)
-            g[t2::kGeoArea + k] = hm.area[4 * c + k];
-        g[t2::kGeoVolume] = hm.volume[c];
-        adj[c] = make_int4(int(hm.adj[4 * c]), int(hm.adj[4 * c + 1]),
-                           int(hm.adj[4 * c + 2]), int(hm.adj[4 * c + 3]));
-    }
-    std::vector<char> sigt(size_t(nc) * ng * ts);
-    for (int64_t i = 0; i < nc * ng; i++) {
-        if (precision == T2_FP64)
-            reinterpret_cast<double *>(sigt.data())[i] = sigma_t[i];
-        else
-            reinterpret_cast<float *>(sigt.data())[i] = float(sigma_t[i]);
-    }
-    const int64_t n_phi = nc * 4 * ng;
-    if ((rc = dalloc_t(s, &s->d_omega, sizeof(double) * 3 * na)) ||
-        (rc = dalloc_t(s, &s->d_wt, sizeof(double) * na)) ||
-        (rc = dalloc_t(s, &s->d_geo, sizeof(double) * geo.size())) ||
-        (rc = dalloc_t(s, &s->d_sigs, sizeof(double) * nc * ng)) ||
-        (rc = dalloc(s, &s->d_sigt, sigt.size())) ||
-        (rc = dalloc_t(s, &s->d_adj, sizeof(int4) * nc)) ||
-        (rc = dalloc_t(s, &s->d_nbr, sizeof(uint32_t) * nc)) ||
-        (rc = dalloc_t(s, &s->d_work, sizeof(uint32_t) * s->sched.work.size())) ||
-        (rc = dalloc(s, &s->d_psi, ts * size_t(na) * n_phi)) ||
-        (rc = dalloc(s, &s->d_src, ts * size_t(n_phi))) ||
-        (rc = dalloc_t(s, &s->d_phi, sizeof(double) * n_phi)) ||
-        (rc = dalloc_t(s, &s->d_phi_old, sizeof(double) * n_phi)) ||
-        (rc = dalloc_t(s, &s->d_red, sizeof(double) * t2::kRedBlocks * 64)) ||
-        (rc = dalloc_t(s, &s->d_scal, sizeof(double) * 64)))
+    rc = setup_device(s, hm, omega, weights, sigma_t, sigma_s);
+    if (rc != T2_OK)
         return bail(rc);
-    const int64_t n_lag = int64_t(s->sched.lagged.size());
-    if (n_lag > 0) {
-        std::vector<int64_t> key(n_lag);
-        std::vector<int32_t> up(n_lag);
-        for (int64_t i = 0; i < n_lag; i++) {
-            const auto &l = s->sched.lagged[i];
-            key[i] = (int64_t(l.angle) * nc + l.cell) * 4 + l.face;
-            up[i] = l.upwind;
-        }
-        if ((rc = dalloc_t(s, &s->d_lag_key, sizeof(int64_t) * n_lag)) ||
-            (rc = dalloc_t(s, &s->d_lag_up, sizeof(int32_t) * n_lag)) ||
-            (rc = dalloc(s, &s->d_lag_buf, ts * size_t(n_lag) * 4 * ng)))
-            return bail(rc);
-        if (hipMemcpy(s->d_lag_key, key.data(), sizeof(int64_t) * n_lag,
-                      hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(s->d_lag_up, up.data(), sizeof(int32_t) * n_lag,
-                      hipMemcpyHostToDevice) != hipSuccess)
-            return bail(fail(T2_ERR_HIP, "upload of lagged faces failed"));
-    }
-    if (hipMemcpy(s->d_omega, omega, sizeof(double) * 3 * na,
-                  hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->d_wt, weights, sizeof(double) * na,
-                  hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->d_geo, geo.data(), sizeof(double) * geo.size(),
-                  hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->d_sigs, sigma_s, sizeof(double) * nc * ng,
-                  hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->d_sigt, sigt.data(), sigt.size(),
-                  hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->d_adj, adj.data(), sizeof(int4) * nc,
-                  hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->d_nbr, hm.nbr_pack.data(), sizeof(uint32_t) * nc,
-                  hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->d_work, s->sched.work.data(),
-                  sizeof(uint32_t) * s->sched.work.size(),
-                  hipMemcpyHostToDevice) != hipSuccess)
-        return bail(fail(T2_ERR_HIP, "upload of set-up data failed"));
-    std::vector<uint32_t>().swap(s->sched.work);  // device copy is enough
-    if ((rc = zero_psi(s)) ||
-        (rc = total_source(s, nullptr, nullptr)))
-        return bail(rc);
-    if (hipMemsetAsync(s->d_phi, 0, sizeof(double) * n_phi, s->stream) !=
-            hipSuccess ||
-        hipStreamSynchronize(s->stream) != hipSuccess)
-        return bail(fail(T2_ERR_HIP, "initialisation failed"));
     *out = s;
     return T2_OK;
 }
@@ -572,6 +641,13 @@ int t2_sweep_step(t2_sweeper *s)
     return T2_OK;
 }
 
+int t2_sweep_only(t2_sweeper *s)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    return run_sweep(s, s->d_qang);
+}
+
 int t2_solve_source_iteration(t2_sweeper *s, int64_t iter_max, double err_max,
                               int64_t *n_iter, double *err_hist)
 {
@@ -652,7 +728,7 @@ int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,
         rnorm = beta;
         if (beta <= target || beta == 0.0)
             break;
-        T2_HIP(t2::launch_axpby(0.0, V, 1.0 / beta, V, n, s->stream));
+        T2_HIP(t2::launch_scale(V, 1.0 / beta, n, s->stream));
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
         int64_t j = 0;
@@ -675,7 +751,7 @@ int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,
             const double hn = std::sqrt(ww);
             H[(j + 1) * m + j] = hn;
             if (hn != 0.0)
-                T2_HIP(t2::launch_axpby(0.0, w, 1.0 / hn, w, n, s->stream));
+                T2_HIP(t2::launch_scale(w, 1.0 / hn, n, s->stream));
             for (int64_t i = 0; i < j; i++) {
                 const double t = cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
                 H[(i + 1) * m + j] =

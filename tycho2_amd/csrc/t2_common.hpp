@@ -5,7 +5,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/tycho2_sweep.h"
+#include "tycho2_sweep.h"
 
 namespace t2 {
 

@@ -143,25 +143,29 @@ int build_host_mesh(int64_t nc, int64_t nn, const double *coords,
 //
 // When the frontier empties before every cell has a level, the remaining
 // cells sit on or behind a dependency cycle (the reference waits forever,
-// src/GraphTraverser.cc:518).  The unfinished cell with the fewest unmet
-// upwind faces (lowest index on ties) is released at the next level and its
-// unmet faces are lagged: during a sweep they read the angular flux the
-// upwind cell held before the sweep.  The set of finished cells at a stall is
-// the same for every traversal order, so the lagged set is the one the
-// oracle's FIFO traversal finds.
+// src/GraphTraverser.cc:518).  The weakest unmet face -- smallest
+// A |Omega.n| over the incoming interior faces of unfinished cells not yet
+// released, lowest (cell, face) on ties -- is lagged: during a sweep it reads
+// the angular flux its upwind cell held before the sweep.  The set of
+// finished cells at a stall is the same for every traversal order, so the
+// lagged set is the one the oracle's FIFO traversal finds.
 void angle_levels(const HostMesh &m, const double *om, int32_t angle,
                   int32_t *level, std::vector<LaggedFace> &lagged,
                   std::vector<int32_t> &indeg, std::vector<int32_t> &frontier)
 {
     const int64_t nc = m.nc;
+    // met[c][k]: 0 unmet dependency, 1 released, 2 lagged, 3 no dependency
+    std::vector<uint8_t> met(4 * nc);
     indeg.assign(nc, 0);
     frontier.clear();
     for (int64_t c = 0; c < nc; c++) {
         int d = 0;
-        for (int k = 0; k < 4; k++)
-            if (m.adj[4 * c + k] >= 0 &&
-                !(omega_dot_n(om, &m.normal[(4 * c + k) * 3]) > 0))
-                d++;
+        for (int k = 0; k < 4; k++) {
+            const bool dep = m.adj[4 * c + k] >= 0 &&
+                             !(omega_dot_n(om, &m.normal[(4 * c + k) * 3]) > 0);
+            met[4 * c + k] = dep ? 0 : 3;
+            d += dep;
+        }
         indeg[c] = d;
         level[c] = -1;
     }
@@ -174,30 +178,49 @@ void angle_levels(const HostMesh &m, const double *om, int32_t angle,
     int32_t lev = 0;
     std::vector<int32_t> next;
     while (done < nc) {
-        if (frontier.empty()) {
-            int64_t best = -1;
-            for (int64_t c = 0; c < nc; c++)
-                if (level[c] < 0 && (best < 0 || indeg[c] < indeg[best]))
-                    best = c;
-            for (int k = 0; k < 4; k++) {
-                int64_t u = m.adj[4 * best + k];
-                if (u >= 0 && level[u] < 0 &&
-                    !(omega_dot_n(om, &m.normal[(4 * best + k) * 3]) > 0))
-                    lagged.push_back({angle, int32_t(best), k, int32_t(u)});
+        while (frontier.empty()) {
+            int64_t bc = -1;
+            int bk = -1;
+            double bw = 0.0;
+            for (int64_t c = 0; c < nc; c++) {
+                if (level[c] >= 0)
+                    continue;
+                for (int k = 0; k < 4; k++) {
+                    if (met[4 * c + k] != 0)
+                        continue;
+                    const double w =
+                        m.area[4 * c + k] *
+                        std::fabs(omega_dot_n(om, &m.normal[(4 * c + k) * 3]));
+                    if (bc < 0 || w < bw) {
+                        bc = c;
+                        bk = k;
+                        bw = w;
+                    }
+                }
             }
-            indeg[best] = 0;
-            level[best] = lev;
-            frontier.push_back(int32_t(best));
+            met[4 * bc + bk] = 2;
+            lagged.push_back({angle, int32_t(bc), bk,
+                              int32_t(m.adj[4 * bc + bk])});
+            if (--indeg[bc] == 0) {
+                level[bc] = lev;
+                frontier.push_back(int32_t(bc));
+            }
         }
         next.clear();
         for (int32_t c : frontier) {
             done++;
             for (int k = 0; k < 4; k++) {
-                int64_t u = m.adj[4 * int64_t(c) + k];
-                if (u < 0 || level[u] >= 0)
+                const int64_t u = m.adj[4 * int64_t(c) + k];
+                if (u < 0 || level[u] >= 0 ||
+                    !(omega_dot_n(om, &m.normal[(4 * int64_t(c) + k) * 3]) > 0))
                     continue;
-                if (omega_dot_n(om, &m.normal[(4 * int64_t(c) + k) * 3]) > 0 &&
-                    --indeg[u] == 0) {
+                int back = 0;  // u's face towards c
+                while (m.adj[4 * u + back] != c)
+                    back++;
+                if (met[4 * u + back] != 0)
+                    continue;
+                met[4 * u + back] = 1;
+                if (--indeg[u] == 0) {
                     level[u] = lev + 1;
                     next.push_back(int32_t(u));
                 }
@@ -318,16 +341,18 @@ int t2_mesh_geometry(int64_t n_cells, int64_t n_nodes, const double *coords,
 }
 
 int t2_sweep_levels(int64_t n_cells, const int64_t *adj_cell,
-                    const double *normal, int64_t n_angles,
-                    const double *omega, int32_t *levels, int64_t *n_lagged)
+                    const double *area, const double *normal,
+                    int64_t n_angles, const double *omega, int32_t *levels,
+                    int64_t *n_lagged)
 {
     t2::clear_error();
-    if (n_cells <= 0 || n_angles <= 0 || !adj_cell || !normal || !omega ||
-        !levels)
+    if (n_cells <= 0 || n_angles <= 0 || !adj_cell || !area || !normal ||
+        !omega || !levels)
         return t2::fail(T2_ERR_ARG, "t2_sweep_levels: bad argument");
     t2::HostMesh m;
     m.nc = n_cells;
     m.adj.assign(adj_cell, adj_cell + 4 * n_cells);
+    m.area.assign(area, area + 4 * n_cells);
     m.normal.assign(normal, normal + 12 * n_cells);
     std::vector<t2::LaggedFace> lagged;
     int rc = t2::build_levels(m, n_angles, omega, levels, lagged);

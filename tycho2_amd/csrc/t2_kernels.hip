@@ -2,9 +2,11 @@
 //
 // The cell solve restates the reference's linear-DG update of one
 // (cell, angle) for all groups (src/SweepData.hh:139-174,
-// src/Transport.cc:54-410, 687-776) in registers: one work item is one
-// (angle, cell); the lanes of a wavefront cover consecutive groups of an
-// item, so every flux load and store touches group-contiguous memory.
+// src/Transport.cc:54-410, 687-776) in registers.  One work item is one
+// (angle, cell); its geometry is precomputed once per mesh into a compact
+// record read as a stream in step order, and the lanes of a wavefront cover
+// consecutive groups of an item, so every flux load and store touches
+// group-contiguous memory.
 #include "t2_kernels.hpp"
 
 namespace t2 {
@@ -22,12 +24,18 @@ __device__ __forceinline__ double omega_dot_n(double ox, double oy, double oz,
     return ox * n[0] + oy * n[1] + oz * n[2];
 }
 
+// Reciprocal: hardware estimate refined by two Newton steps in FP64 (within
+// an ulp or so of 1/x), the hardware estimate alone in FP32.
 template <typename T>
 __device__ __forceinline__ T recip(T x);
 template <>
 __device__ __forceinline__ double recip<double>(double x)
 {
-    return 1.0 / x;
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
 }
 template <>
 __device__ __forceinline__ float recip<float>(float x)
@@ -51,7 +59,7 @@ __device__ __forceinline__ int64_t find_key(const int64_t *keys, int64_t n,
 }
 
 // 4x4 solve by row elimination without pivoting (src/Transport.cc:338-410):
-// rows 0..2 scaled by their pivot's reciprocal, last pivot divides.
+// rows 0..2 scaled by their pivot's reciprocal, then back substitution.
 template <typename T>
 __device__ __forceinline__ void solve_nopivot(T m[4][4], T b[4])
 {
@@ -71,7 +79,7 @@ __device__ __forceinline__ void solve_nopivot(T m[4][4], T b[4])
             b[row] -= b[r] * t;
         }
     }
-    b[3] = b[3] / m[3][3];
+    b[3] *= recip<T>(m[3][3]);
 #pragma unroll
     for (int r = 2; r >= 0; r--)
 #pragma unroll
@@ -79,105 +87,153 @@ __device__ __forceinline__ void solve_nopivot(T m[4][4], T b[4])
             b[r] -= m[r][c] * b[c];
 }
 
+// ---------------------------------------------------------------- set-up
+template <typename T>
+__global__ void build_items_kernel(const uint32_t *__restrict__ work,
+                                   int64_t n, int cell_bits, int64_t nc,
+                                   const double *__restrict__ omega,
+                                   const double *__restrict__ geo,
+                                   const int4 *__restrict__ adj,
+                                   const uint32_t *__restrict__ nbr,
+                                   const double *__restrict__ sigma_cell,
+                                   const int64_t *__restrict__ lag_key,
+                                   int64_t n_lag, ItemGeo<T> *out_geo,
+                                   ItemTopo *out_topo, int *err)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t w = work[i];
+    const int64_t c = int64_t(w & ((1u << cell_bits) - 1u));
+    const int64_t a = int64_t(w >> cell_bits);
+    const double ox = omega[3 * a], oy = omega[3 * a + 1],
+                 oz = omega[3 * a + 2];
+    const double *g = geo + c * kGeoStride;
+    const int4 nb4 = adj[c];
+    const int32_t nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    const uint32_t np = nbr[c];
+    ItemGeo<T> rg;
+    ItemTopo rt;
+    rt.cell = int32_t(c);
+    rt.angle = int32_t(a);
+    rt.meta = 0u;
+    rt.pad = 0;
+    int n_in = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        // Abar_k = A_k (Omega . n_k)  (src/Transport.cc:697-704)
+        const double abar = g[kGeoArea + k] * omega_dot_n(ox, oy, oz, g + 3 * k);
+        rg.a12[k] = T(abar * kInv12);
+        rt.up[k] = -1;
+        if (abar < 0.0 && nb[k] >= 0) {
+            n_in++;
+            const int64_t lag =
+                n_lag > 0 ? find_key(lag_key, n_lag, (a * nc + c) * 4 + k) : -1;
+            rt.up[k] = lag >= 0 ? int32_t(-2 - lag) : nb[k];
+            int slot = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (j == k)
+                    continue;
+                const uint32_t vtx = (np >> (2 * (4 * k + j))) & 3u;
+                rt.meta |= vtx << (6 * k + 2 * slot);
+                slot++;
+            }
+        }
+    }
+    rg.v20 = T(g[kGeoVolume] * kInv20);
+    rg.sig = sigma_cell ? T(sigma_cell[c]) : T(0);
+    if (n_in > 3)
+        *err = 1;
+    out_geo[i] = rg;
+    out_topo[i] = rt;
+}
+
+// ---------------------------------------------------------------- sweep
 template <typename T, int G, int GPL>
 __global__ void __launch_bounds__(kMaxBlock)
 sweep_step_kernel(const SweepArgs<T> p)
 {
-    constexpr int LPI = G / GPL;  // lanes per work item
+    constexpr int LPI = G / GPL;          // lanes per work item
+    constexpr uint32_t CG = 4 * G;        // values per (cell, angle)
     const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t item = tid / LPI;
     const int gl = int(tid % LPI);
     if (item >= p.n_items)
         return;
-    const uint32_t w = p.work[item];
-    const int64_t c = int64_t(w & ((1u << p.cell_bits) - 1u));
-    const int64_t a = int64_t(w >> p.cell_bits);
-    const double ox = p.omega[3 * a], oy = p.omega[3 * a + 1],
-                 oz = p.omega[3 * a + 2];
-    const double *geo = p.geo + c * kGeoStride;
-
-    // Abar_k = A_k (Omega . n_k) (src/Transport.cc:697-704); the tests below
-    // are the reference's incoming / outgoing rules (Transport.cc:113,184).
-    bool in_k[4], out_k[4];
-    T a12[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const double abar =
-            geo[kGeoArea + k] * omega_dot_n(ox, oy, oz, geo + 3 * k);
-        in_k[k] = abar < 0.0;
-        out_k[k] = abar > 0.0;
-        a12[k] = T(abar * kInv12);
-    }
-    const T v20 = T(geo[kGeoVolume] * kInv20);
-    const int4 nb4 = p.adj[c];
-    const int32_t nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
-    const uint32_t np = p.nbr[c];
+    const ItemTopo t = p.topo[item];
+    const ItemGeo<T> ge = p.geo[item];
+    const size_t plane = size_t(p.nc) * CG;
+    const T *psi_a = p.psi + size_t(t.angle) * plane;
+    const uint32_t own = uint32_t(t.cell) * CG;
 
     // outgoing-face sums of the streaming matrix
-    T P = T(0);
     T o[4];
+    T P = T(0);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        o[k] = out_k[k] ? a12[k] : T(0);
+        o[k] = ge.a12[k] > T(0) ? ge.a12[k] : T(0);
         P += o[k];
-    }
-
-    const size_t plane = size_t(p.nc) * 4 * G;  // one angle of psi
-    const T *psi_a = p.psi + size_t(a) * plane;
-    int64_t lag_at[4] = {-1, -1, -1, -1};
-    if (p.n_lag > 0) {
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-            if (in_k[k] && nb[k] >= 0)
-                lag_at[k] = find_key(p.lag_key, p.n_lag, (a * p.nc + c) * 4 + k);
     }
 
 #pragma unroll
     for (int jg = 0; jg < GPL; jg++) {
         const int g = gl + jg * LPI;
-        const T s = p.sigma_t[c * G + g] * v20;
+        const T sig = p.sigma_mode == kSigmaGroup ? p.sigma[g]
+                      : p.sigma_mode == kSigmaCell
+                          ? ge.sig
+                          : p.sigma[uint32_t(t.cell) * G + g];
+        const T s = sig * ge.v20;
 
         // source: V/20 (q_i + sum_j q_j)  (Transport.cc:58-72)
         T q[4];
 #pragma unroll
         for (int v = 0; v < 4; v++)
-            q[v] = p.src_iso[(c * 4 + v) * G + g];
+            q[v] = p.src_iso[own + v * G + g];
         if (p.src_ang) {
-            const double *qa = p.src_ang + size_t(a) * plane;
+            const double *qa = p.src_ang + size_t(t.angle) * plane + own + g;
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                q[v] += T(qa[(c * 4 + v) * G + g]);
+                q[v] += T(qa[v * G]);
         }
         const T sumq = (q[0] + q[1]) + (q[2] + q[3]);
         T b[4];
 #pragma unroll
         for (int v = 0; v < 4; v++)
-            b[v] = v20 * (sumq + q[v]);
+            b[v] = ge.v20 * (sumq + q[v]);
 
-        // upwind faces: b_i -= a_k (sum_j psi_j + psi_i), j, i != k
+        // upwind faces: b_i -= a_k (sum_j psi_j + psi_i), i, j != k
         // (Transport.cc:175-271, 740-776); vacuum faces contribute zero
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            if (!in_k[k] || nb[k] < 0)
+            const int32_t u = t.up[k];
+            if (u == -1)
                 continue;
-            T u[4];
-            if (lag_at[k] < 0) {
-                const T *pn = psi_a + size_t(nb[k]) * 4 * G + g;
+            T uv[4];
+            if (u >= 0) {
+                const T *pn = psi_a + uint32_t(u) * CG + g;
+                int slot = 0;
 #pragma unroll
-                for (int j = 0; j < 4; j++)
-                    u[j] = (j == k) ? T(0)
-                                    : pn[((np >> (2 * (4 * k + j))) & 3u) * G];
+                for (int j = 0; j < 4; j++) {
+                    if (j == k) {
+                        uv[j] = T(0);
+                        continue;
+                    }
+                    const uint32_t vtx = (t.meta >> (6 * k + 2 * slot)) & 3u;
+                    uv[j] = pn[vtx * G];
+                    slot++;
+                }
             } else {
-                const T *pl = p.lag_buf + size_t(lag_at[k]) * 4 * G + g;
+                const T *pl = p.lag_buf + size_t(-2 - u) * CG + g;
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    u[j] = (j == k) ? T(0) : pl[j * G];
+                    uv[j] = (j == k) ? T(0) : pl[j * G];
             }
-            const T sum = (u[0] + u[1]) + (u[2] + u[3]);
+            const T sum = (uv[0] + uv[1]) + (uv[2] + uv[3]);
 #pragma unroll
             for (int i = 0; i < 4; i++)
                 if (i != k)
-                    b[i] -= a12[k] * (sum + u[i]);
+                    b[i] -= ge.a12[k] * (sum + uv[i]);
         }
 
         // M[i][j] = a_i + (1 + d_ij)(s + sum_{k out, k != i, j} a_k)
@@ -187,11 +243,11 @@ sweep_step_kernel(const SweepArgs<T> p)
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++)
-                m[i][j] = (i == j) ? a12[i] + T(2) * (s + (P - o[i]))
-                                   : a12[i] + s + ((P - o[i]) - o[j]);
+                m[i][j] = (i == j) ? ge.a12[i] + T(2) * (s + (P - o[i]))
+                                   : ge.a12[i] + s + ((P - o[i]) - o[j]);
         solve_nopivot<T>(m, b);
 
-        T *dst = p.psi + size_t(a) * plane + size_t(c) * 4 * G + g;
+        T *dst = p.psi + size_t(t.angle) * plane + own + g;
 #pragma unroll
         for (int v = 0; v < 4; v++)
             dst[v * G] = b[v];
@@ -261,17 +317,35 @@ __global__ void lag_gather_kernel(const int64_t *lag_key,
 
 // ---------------------------------------------------------------- moments
 template <typename T>
+struct alignas(16) Vec16 {
+    T v[16 / sizeof(T)];
+};
+
+template <typename T>
 __global__ void psi_to_phi_kernel(const T *__restrict__ psi,
                                   const double *__restrict__ wt, int64_t na,
                                   int64_t n_phi, double *__restrict__ phi)
 {
 #pragma clang fp contract(off)
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-         i < n_phi; i += int64_t(gridDim.x) * blockDim.x) {
-        double acc = 0.0;
-        for (int64_t a = 0; a < na; a++)
-            acc += double(psi[size_t(a) * n_phi + i]) * wt[a];
-        phi[i] = acc;
+    constexpr int V = 16 / sizeof(T);
+    const int64_t nv = n_phi / V;
+    const Vec16<T> *src = reinterpret_cast<const Vec16<T> *>(psi);
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nv;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        double acc[V];
+#pragma unroll
+        for (int e = 0; e < V; e++)
+            acc[e] = 0.0;
+        for (int64_t a = 0; a < na; a++) {
+            const Vec16<T> x = src[size_t(a) * nv + i];
+            const double w = wt[a];
+#pragma unroll
+            for (int e = 0; e < V; e++)
+                acc[e] += double(x.v[e]) * w;
+        }
+#pragma unroll
+        for (int e = 0; e < V; e++)
+            phi[i * V + e] = acc[e];
     }
 }
 
@@ -421,6 +495,13 @@ __global__ void axpby_kernel(double alpha, const double *__restrict__ x,
         y[i] = alpha * x[i] + beta * y[i];
 }
 
+__global__ void scale_kernel(double *x, double alpha, int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+        x[i] *= alpha;
+}
+
 __global__ void fill_kernel(double *x, double v, int64_t n)
 {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -439,6 +520,24 @@ inline unsigned stream_grid(int64_t n, int block = 256)
 }
 
 }  // namespace
+
+template <typename T>
+hipError_t launch_build_items(const uint32_t *work, int64_t n, int cell_bits,
+                              int64_t nc, const double *omega,
+                              const double *geo, const int4 *adj,
+                              const uint32_t *nbr, const double *sigma_cell,
+                              const int64_t *lag_key, int64_t n_lag,
+                              ItemGeo<T> *out_geo, ItemTopo *out_topo,
+                              int *err, hipStream_t stream)
+{
+    if (n <= 0)
+        return hipSuccess;
+    const unsigned grid = unsigned((n + 255) / 256);
+    hipLaunchKernelGGL((build_items_kernel<T>), dim3(grid), dim3(256), 0,
+                       stream, work, n, cell_bits, nc, omega, geo, adj, nbr,
+                       sigma_cell, lag_key, n_lag, out_geo, out_topo, err);
+    return hipGetLastError();
+}
 
 template <typename T>
 hipError_t launch_sweep_step(const SweepArgs<T> &args, int groups,
@@ -476,7 +575,8 @@ template <typename T>
 hipError_t launch_psi_to_phi(const T *psi, const double *wt, int64_t na,
                              int64_t n_phi, double *phi, hipStream_t stream)
 {
-    hipLaunchKernelGGL((psi_to_phi_kernel<T>), dim3(stream_grid(n_phi)),
+    const int64_t nv = n_phi / (16 / sizeof(T));
+    hipLaunchKernelGGL((psi_to_phi_kernel<T>), dim3(stream_grid(nv)),
                        dim3(256), 0, stream, psi, wt, na, n_phi, phi);
     return hipGetLastError();
 }
@@ -552,6 +652,14 @@ hipError_t launch_axpby(double alpha, const double *x, double beta, double *y,
     return hipGetLastError();
 }
 
+hipError_t launch_scale(double *x, double alpha, int64_t n,
+                        hipStream_t stream)
+{
+    hipLaunchKernelGGL(scale_kernel, dim3(stream_grid(n)), dim3(256), 0,
+                       stream, x, alpha, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t stream)
 {
     hipLaunchKernelGGL(fill_kernel, dim3(stream_grid(n)), dim3(256), 0,
@@ -559,34 +667,29 @@ hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t stream)
     return hipGetLastError();
 }
 
-template hipError_t launch_sweep_step<double>(const SweepArgs<double> &, int,
-                                              int, int, hipStream_t);
-template hipError_t launch_sweep_step<float>(const SweepArgs<float> &, int,
-                                             int, int, hipStream_t);
-template hipError_t launch_lag_gather<double>(const int64_t *, const int32_t *,
-                                              int64_t, int64_t, int,
-                                              const uint32_t *, const double *,
-                                              double *, hipStream_t);
-template hipError_t launch_lag_gather<float>(const int64_t *, const int32_t *,
-                                             int64_t, int64_t, int,
-                                             const uint32_t *, const float *,
-                                             float *, hipStream_t);
-template hipError_t launch_psi_to_phi<double>(const double *, const double *,
-                                              int64_t, int64_t, double *,
-                                              hipStream_t);
-template hipError_t launch_psi_to_phi<float>(const float *, const double *,
-                                             int64_t, int64_t, double *,
-                                             hipStream_t);
-template hipError_t launch_total_source<double>(const double *,
-                                                const double *,
-                                                const double *, int64_t, int,
-                                                double *, hipStream_t);
-template hipError_t launch_total_source<float>(const double *, const double *,
-                                               const double *, int64_t, int,
-                                               float *, hipStream_t);
-template hipError_t launch_ac_to_ca<double>(const double *, double *, int64_t,
-                                            int64_t, int64_t, hipStream_t);
-template hipError_t launch_ac_to_ca<float>(const float *, double *, int64_t,
+#define T2_INSTANTIATE(T)                                                     \
+    template hipError_t launch_build_items<T>(                                \
+        const uint32_t *, int64_t, int, int64_t, const double *,              \
+        const double *, const int4 *, const uint32_t *, const double *,       \
+        const int64_t *, int64_t, ItemGeo<T> *, ItemTopo *, int *,            \
+        hipStream_t);                                                         \
+    template hipError_t launch_sweep_step<T>(const SweepArgs<T> &, int, int,  \
+                                             int, hipStream_t);               \
+    template hipError_t launch_lag_gather<T>(const int64_t *, const int32_t *,\
+                                             int64_t, int64_t, int,           \
+                                             const uint32_t *, const T *,     \
+                                             T *, hipStream_t);               \
+    template hipError_t launch_psi_to_phi<T>(const T *, const double *,       \
+                                             int64_t, int64_t, double *,      \
+                                             hipStream_t);                    \
+    template hipError_t launch_total_source<T>(const double *,                \
+                                               const double *,                \
+                                               const double *, int64_t, int,  \
+                                               T *, hipStream_t);             \
+    template hipError_t launch_ac_to_ca<T>(const T *, double *, int64_t,      \
                                            int64_t, int64_t, hipStream_t);
+
+T2_INSTANTIATE(double)
+T2_INSTANTIATE(float)
 
 }  // namespace t2

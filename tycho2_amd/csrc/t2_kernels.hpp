@@ -7,8 +7,8 @@
 
 namespace t2 {
 
-// Per-cell geometry record on the device: 4 outward unit normals, 4 face
-// areas, the volume, padded to 160 bytes.
+// Per-cell geometry record on the device (set-up only): 4 outward unit
+// normals, 4 face areas, the volume, padded to 160 bytes.
 constexpr int kGeoStride = 20;
 constexpr int kGeoNormal = 0;
 constexpr int kGeoArea = 12;
@@ -17,26 +17,59 @@ constexpr int kGeoVolume = 16;
 // Largest block of the sweep kernels (their __launch_bounds__).
 constexpr int kMaxBlock = 256;
 
+// One work item = one (angle, cell) of the sweep, stored in step order and
+// read as a stream by the sweep kernels.
+//   a12[k] = A_k (Omega . n_k) / 12: < 0 incoming, > 0 outgoing, 0 neither
+//   v20    = V / 20
+//   sig    = sigma_t of the cell (used when sigma_t does not vary by group)
+template <typename T>
+struct alignas(16) ItemGeo {
+    T a12[4];
+    T v20;
+    T sig;
+};
+
+// up[k]: upwind cell through face k (>= 0), -1 when face k is outgoing,
+// parallel or on the vacuum boundary, or -2 - i for lagged face i (its
+// values come from the pre-sweep snapshot lag_buf[i]).  meta holds, for each
+// face k, the neighbour vertex facing cell vertex j (j != k, ascending) in
+// bits 6k + 2*slot.
+struct alignas(16) ItemTopo {
+    int32_t cell;
+    int32_t angle;
+    int32_t up[4];
+    uint32_t meta;
+    int32_t pad;
+};
+
+enum SigmaMode { kSigmaCellGroup = 0, kSigmaGroup = 1, kSigmaCell = 2 };
+
 template <typename T>
 struct SweepArgs {
-    const uint32_t *work;      // step slice of the work list
+    const ItemGeo<T> *geo;     // step slice of the item records
+    const ItemTopo *topo;
     int64_t n_items;
-    int cell_bits;
     int64_t nc;
-    const double *omega;       // [na][3]
-    const double *geo;         // [nc][kGeoStride]
-    const int4 *adj;           // [nc] neighbour per face, -1 on the boundary
-    const uint32_t *nbr;       // [nc] packed neighbour vertices
-    const T *sigma_t;          // [nc][G]
+    int sigma_mode;
+    const T *sigma;            // [nc][G] (cell-group) or [G] (group)
     const T *src_iso;          // [nc][4][G] isotropic total source
     const double *src_ang;     // [na][nc][4][G] angle-dependent part or null
     T *psi;                    // [na][nc][4][G]
-    const int64_t *lag_key;    // sorted ((a*nc + c)*4 + k), or null
-    int64_t n_lag;
     const T *lag_buf;          // [n_lag][4][G] upwind values held pre-sweep
 };
 
-// Sweep of one step: every work item's cell solve for all groups.
+// Builds the item records from the step-ordered work list; sets *err to a
+// nonzero value if an item has more than three incoming interior faces.
+template <typename T>
+hipError_t launch_build_items(const uint32_t *work, int64_t n, int cell_bits,
+                              int64_t nc, const double *omega,
+                              const double *geo, const int4 *adj,
+                              const uint32_t *nbr, const double *sigma_cell,
+                              const int64_t *lag_key, int64_t n_lag,
+                              ItemGeo<T> *out_geo, ItemTopo *out_topo,
+                              int *err, hipStream_t stream);
+
+// Sweep of one step: every item's cell solve for all groups.
 template <typename T>
 hipError_t launch_sweep_step(const SweepArgs<T> &args, int groups,
                              int groups_per_lane, int block,
@@ -83,9 +116,12 @@ hipError_t launch_multi_dot(const double *V, int64_t ldv, int m,
 hipError_t launch_multi_axpy(const double *V, int64_t ldv, int m,
                              const double *h, double *w, int64_t n,
                              hipStream_t stream);
-// y = alpha * x + beta * y
+// y = alpha * x + beta * y  (x and y distinct)
 hipError_t launch_axpby(double alpha, const double *x, double beta, double *y,
                         int64_t n, hipStream_t stream);
+// x = alpha * x
+hipError_t launch_scale(double *x, double alpha, int64_t n,
+                        hipStream_t stream);
 hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t stream);
 
 }  // namespace t2

@@ -33,7 +33,7 @@ SIGNATURES = {
     "t2_abi_version": (ctypes.c_int, []),
     "t2_mesh_geometry": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, _dp,
                                         _ip, _ip, _dp, _dp, _dp]),
-    "t2_sweep_levels": (ctypes.c_int, [ctypes.c_int64, _ip, _dp,
+    "t2_sweep_levels": (ctypes.c_int, [ctypes.c_int64, _ip, _dp, _dp,
                                        ctypes.c_int64, _dp, _i32p, _ip]),
     "t2_sweeper_create": (ctypes.c_int, [
         ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int64,
@@ -53,6 +53,7 @@ SIGNATURES = {
         ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int64,
         _ip, _dp]),
     "t2_sweep_step": (ctypes.c_int, [ctypes.c_void_p]),
+    "t2_sweep_only": (ctypes.c_int, [ctypes.c_void_p]),
     "t2_get_phi": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "t2_get_psi": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "t2_set_phi": (ctypes.c_int, [ctypes.c_void_p, _dp]),
@@ -131,14 +132,15 @@ def mesh_geometry(smesh, conn):
     return vol, area, nrm
 
 
-def sweep_levels(adj, normal, omega):
+def sweep_levels(adj, area, normal, omega):
     """(levels[a][c], n_lagged) of the engine's per-angle schedule."""
-    adj, nrm, om = _i64(adj), _f64(normal), _f64(omega)
+    adj, ar, nrm, om = _i64(adj), _f64(area), _f64(normal), _f64(omega)
     nc, na = adj.shape[0], om.shape[0]
     levels = np.empty((na, nc), dtype=np.int32)
     lag = ctypes.c_int64(0)
-    _check(lib().t2_sweep_levels(nc, _ptr(adj, _ip), _ptr(nrm), na, _ptr(om),
-                                 _ptr(levels, _i32p), ctypes.byref(lag)))
+    _check(lib().t2_sweep_levels(nc, _ptr(adj, _ip), _ptr(ar), _ptr(nrm), na,
+                                 _ptr(om), _ptr(levels, _i32p),
+                                 ctypes.byref(lag)))
     return levels, lag.value
 
 
@@ -245,6 +247,9 @@ class SweeperHIP:
 
     def sweep_step(self) -> None:
         _check(lib().t2_sweep_step(self.handle))
+
+    def sweep_only(self) -> None:
+        _check(lib().t2_sweep_only(self.handle))
 
     def synchronize(self) -> None:
         _check(lib().t2_synchronize(self.handle))
