@@ -112,6 +112,13 @@ int t2_sweeper_info(const t2_sweeper *s, t2_info *info);
  * 0 keeps the default. */
 int t2_set_launch(t2_sweeper *s, int items_per_block, int groups_per_lane);
 
+/* Sweep schedule: the angles are swept in `angle_batches` consecutive
+ * batches (octant by octant, then by index), each batch level by level.  One
+ * batch runs every angle concurrently; more batches shrink each step's
+ * working set so that upwind reads stay in cache (autotuner knob).  Results
+ * do not depend on it. */
+int t2_set_schedule(t2_sweeper *s, int angle_batches);
+
 /* ------------------------------------------------------------ drop-in sweep
  * SweeperAbstract::sweep(psi, source, zeroPsiBound) on host arrays in the
  * reference layout: psi = L^-1 source for all angles and groups.  psi is
@@ -140,6 +147,31 @@ int t2_solve_source_iteration(t2_sweeper *s, int64_t iter_max, double err_max,
  * res_hist (host, length iter_max) may be NULL. */
 int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,
                     int64_t restart, int64_t *n_iter, double *res_hist);
+
+/* Mixed-precision solve (handles created with T2_FP32): FP32 sweeps with an
+ * FP32 angular flux inside FP64 defect correction.
+ *   1. GMRES(restart) in FP64 on phi with FP32 sweeps, to rtol_sweep;
+ *      psi32 = L^-1 (q + M S phi) by one more sweep, phi1 = D psi32 (FP64).
+ *   2. r = q + M S phi1 - L psi32, the defect of the coupled equations,
+ *      evaluated in FP64 from FP64 cell coefficients (a parallel operator
+ *      apply, no sweep), stored in FP32.
+ *   3. GMRES on the correction (L - M S D) delta = r with FP32 sweeps, to
+ *      rtol_correction; phi = phi1 + D delta in FP64.
+ * Memory: psi32 plus an FP32 residual (psi-sized each).  phi1 (host,
+ * [cell][vertex][group], may be NULL) receives the scalar flux after step 1
+ * so callers can report the error of each step. */
+typedef struct {
+    int64_t iters_sweep;       /* GMRES iterations of step 1              */
+    int64_t iters_correction;  /* GMRES iterations of step 3              */
+    int64_t sweeps;            /* FP32 sweeps in total                    */
+    double residual_sweep;     /* relative GMRES residual after step 1    */
+    double residual_correction;
+    double correction_rel;     /* max|D delta| / max|phi|                 */
+} t2_refine_info;
+
+int t2_solve_refined(t2_sweeper *s, int64_t iter_max, double rtol_sweep,
+                     double rtol_correction, int64_t restart,
+                     t2_refine_info *info, double *phi1);
 
 /* One sweep with the current scalar flux: psi = L^-1 (q + S phi) and
  * phi <- D psi (one source-iteration step; the benchmark's unit of work). */

@@ -1,0 +1,65 @@
+"""The C++ host driver sweep_hip.x: the reference's executable shape with
+SweepType HIP, SweeperAbstract::sweep driven by a host source iteration."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tycho2_amd import build, psi_io
+
+from conftest import golden
+
+GOLD_DECK = """# input-gold.deck of the reference with SweepType HIP
+snOrder         8
+iterMax         100
+errMax          1e-10
+maxCellsPerStep 100
+nGroups         2
+sigmaT1         10
+sigmaS1         5
+sigmaT2         10
+sigmaS2         5
+OutputFile      true
+OutputFilename  {out}
+SourceIteration true
+MPIType         TychoTwoSided
+SweepType       HIP
+GaussElim       NoPivot
+{extra}
+"""
+
+
+def run(deck_text, tmp_path, mesh=None):
+    deck = tmp_path / "input.deck"
+    deck.write_text(deck_text)
+    return subprocess.run([build.DRIVER, mesh or golden("cube-208.smesh"),
+                           str(deck)], capture_output=True, text=True,
+                          timeout=300)
+
+
+def test_driver_usage_and_errors(tmp_path):
+    r = subprocess.run([build.DRIVER], capture_output=True, text=True)
+    assert r.returncode == 0 and "Usage" in r.stdout
+    out = str(tmp_path / "o.psi")
+    r = run(GOLD_DECK.format(out=out, extra="").replace(
+        "SweepType       HIP", "SweepType       PBJ"), tmp_path)
+    assert r.returncode == 1 and "Sweep type not recognized" in r.stderr
+    r = run(GOLD_DECK.format(out=out, extra="").replace("snOrder         8",
+                                                        ""), tmp_path)
+    assert r.returncode == 1 and "snOrder" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("extra", ["DeviceLoop false", "DeviceLoop true"])
+def test_driver_reproduces_gold(tmp_path, gold, extra):
+    print(f"driver gold {extra}", flush=True)
+    out = str(tmp_path / "out.psi")
+    r = run(GOLD_DECK.format(out=out, extra=extra), tmp_path)
+    assert r.returncode == 0, r.stderr
+    psi = psi_io.read_psi(out)
+    assert psi.shape == (208, 128, 4, 2)
+    got = psi[..., 0][:, gold["psi_angles"], :]
+    l1, linf = psi_io.rel_diff(got, gold["psi_g0"])
+    assert l1 < 1e-10 and linf < 1e-10
+    assert "iterations: 34" in r.stdout

@@ -166,3 +166,88 @@ def test_full_size_plain_angle_planes():
         ref, _ = orc.sweep(m, p.omega[a:a + 1], p.ng, p.sigma_t,
                            tot[:, None])
         assert maxrel(psi_a, ref[:, 0]) <= 1e-12
+
+
+def test_schedule_batches_bitwise():
+    """Sweeping the angles in batches changes the order, not the result."""
+    print("schedule batches", flush=True)
+    ng = 4
+    sm, conn, omega, wt, st, ss, rng = random_problem(n=4, sn=4, ng=ng, seed=6)
+    src = rng.uniform(0.0, 1.0, (sm.n_cells, omega.shape[0], 4, ng))
+    outs = []
+    with SweeperHIP(sm, conn, omega, wt, ng, st, ss, FP64) as s:
+        for batches in (1, 2, 8, omega.shape[0]):
+            s.set_schedule(batches)
+            psi = np.zeros_like(src)
+            s.sweep(psi, src)
+            outs.append(psi)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
+def cyclic_problem():
+    # S2 directions run along the Kuhn diagonal faces: cycles appear
+    sm = meshmod.kuhn_cube(3, 0.2, seed=3)
+    conn = meshmod.tycho_connectivity(sm)
+    omega, wt = orc.quadrature(2)
+    rng = np.random.default_rng(8)
+    st = rng.uniform(1.0, 2.0, (sm.n_cells, 1))
+    return sm, conn, omega, wt, st, 0.5 * st, rng
+
+
+def test_cyclic_mesh_lagged_sweep_matches_oracle():
+    """Lagged faces read the previous psi in both implementations."""
+    print("cyclic sweep", flush=True)
+    sm, conn, omega, wt, st, ss, rng = cyclic_problem()
+    src = rng.uniform(0.1, 1.0, (sm.n_cells, omega.shape[0], 4, 1))
+    prev = rng.uniform(0.0, 1.0, src.shape)
+    psi = prev.copy()
+    with SweeperHIP(sm, conn, omega, wt, 1, st, ss, FP64) as s:
+        n_lag = s.info()["n_lagged"]
+        s.sweep(psi, src)
+    ref, lag = orc.sweep(orc.build_mesh(sm), omega, 1, st, src,
+                         psi=prev.copy())
+    assert n_lag == lag > 0
+    assert maxrel(psi, ref) <= 1e-12
+
+
+def test_cyclic_mesh_source_iteration():
+    print("cyclic SI", flush=True)
+    sm, conn, omega, wt, st, ss, rng = cyclic_problem()
+    q = np.ones((sm.n_cells, 4, 1))
+    src = orc.isotropic_source(sm.n_cells, omega.shape[0], 1,
+                               np.ones((sm.n_cells, 1)))
+    with SweeperHIP(sm, conn, omega, wt, 1, st, ss, FP64) as s:
+        s.set_source_isotropic(q)
+        hist = s.solve_source_iteration(500, 1e-13)
+        phi = s.phi()
+    _, rphi, rhist = orc.source_iteration(orc.build_mesh(sm), omega, wt, 1,
+                                          st, ss, src, 500, 1e-13)
+    assert len(hist) == len(rhist)
+    assert maxrel(phi, rphi) <= 1e-12
+
+
+def test_refined_solve_reaches_1e10():
+    """FP32 sweeps with FP32 psi inside FP64 defect correction: the scalar
+    flux within 1e-10 (max-relative) of the FP64 oracle, where the FP32
+    stage alone is not."""
+    print("refined solve", flush=True)
+    ng = 4
+    sm = meshmod.kuhn_cube(6, 0.15, seed=21)
+    conn = meshmod.tycho_connectivity(sm)
+    omega, wt = product_quadrature(4)
+    st = np.broadcast_to(1.0 + 0.1 * np.arange(ng), (sm.n_cells, ng))
+    ss = 0.9 * st
+    q = np.ones((sm.n_cells, 4, ng))
+    src = orc.isotropic_source(sm.n_cells, omega.shape[0], ng,
+                               np.ones((sm.n_cells, ng)))
+    _, rphi, _ = orc.krylov(orc.build_mesh(sm), omega, wt, ng, st, ss, src,
+                            400, 1e-14, 30)
+    with SweeperHIP(sm, conn, omega, wt, ng, st, ss, FP32) as s:
+        s.set_source_isotropic(q)
+        info, phi1 = s.solve_refined(300, 1e-6, 1e-5, 20, want_phi1=True)
+        phi = s.phi()
+    e1, e2 = maxrel(phi1, rphi), maxrel(phi, rphi)
+    print(f"refined: FP32 stage {e1:.2e}, after correction {e2:.2e}, {info}")
+    assert e2 <= 1e-10
+    assert e1 > 10 * e2

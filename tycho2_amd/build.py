@@ -16,6 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBNAME = "libtycho2_sweep.so"
 LIBPATH = os.path.join(HERE, LIBNAME)
+DRIVER = os.path.join(HERE, "sweep_hip.x")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
@@ -61,6 +62,10 @@ def build(verbose: bool = False) -> str:
     _run([_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp,
           *objs, "-lpthread"])
     shutil.move(tmp, LIBPATH)
+    # C++ host driver (the reference's executable with SweepType HIP)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}",
+          os.path.join(HERE, "host", "sweep_hip.cpp"), "-o", DRIVER,
+          f"-L{HERE}", "-ltycho2_sweep", "-Wl,-rpath,$ORIGIN"])
     return LIBPATH
 
 

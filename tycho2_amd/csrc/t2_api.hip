@@ -30,6 +30,13 @@ using t2::fail;
             return rc_;     \
     } while (0)
 
+// A captured sweep, keyed by the angle-dependent sources it reads.
+struct GraphEntry {
+    const double *ang;     // FP64 angle-dependent source or null
+    const void *ang_t;     // psi-precision angle-dependent source or null
+    hipGraphExec_t exec;
+};
+
 }  // namespace
 
 struct t2_sweeper {
@@ -37,36 +44,43 @@ struct t2_sweeper {
     hipStream_t stream = nullptr;
     int64_t nc = 0, na = 0, ng = 0;
     int precision = T2_FP64, gauss = T2_GAUSS_NOPIVOT;
-    int block = 256, gpl = 1;
+    int block = 256, gpl = 1, batches = 1;
     bool use_graph = true;
+    t2::HostMesh hm;                 // kept for schedule rebuilds
+    std::vector<double> omega_h, sigma_t_h;
     t2::Schedule sched;
     size_t bytes = 0;
     std::vector<void *> allocs;
 
+    // static device data
     double *d_omega = nullptr, *d_wt = nullptr;
-    double *d_sigs = nullptr;
+    double *d_sigs = nullptr;        // FP64 [nc][G]
     int sigma_mode = t2::kSigmaCellGroup;
-    void *d_sigt = nullptr;          // T [nc][G] or [G] (by sigma_mode)
+    void *d_sigt = nullptr;          // T, layout by sigma_mode
+    double *d_sigt64 = nullptr;      // FP64 copy (FP32 handles: residual)
     uint32_t *d_nbr = nullptr;
-    void *d_item_geo = nullptr;      // ItemGeo<T> [nc * na], step order
+    // schedule-dependent
+    void *d_item_geo = nullptr;      // ItemGeo<T> in step order
+    t2::ItemGeo<double> *d_item_geo64 = nullptr;  // FP32 handles: residual
     t2::ItemTopo *d_item_topo = nullptr;
+    int64_t *d_lag_key = nullptr;
+    int32_t *d_lag_up = nullptr;
+    void *d_lag_buf = nullptr;
+    // fluxes and sources
     void *d_psi = nullptr;           // T [na][nc][4][G]
     void *d_src = nullptr;           // T [nc][4][G]
+    double *d_src64 = nullptr;       // FP64 [nc][4][G] (residual)
+    float *d_res = nullptr;          // FP32 [na][nc][4][G] (residual)
     double *d_qiso = nullptr;        // FP64 [nc][4][G]
     double *d_qang = nullptr;        // FP64 [na][nc][4][G]
     double *d_stage = nullptr;       // FP64 psi-sized staging (host path)
     double *d_hostang = nullptr;     // FP64 [na][nc][4][G] (host path)
     double *d_phi = nullptr, *d_phi_old = nullptr;
     double *d_red = nullptr, *d_scal = nullptr;
-    int64_t *d_lag_key = nullptr;
-    int32_t *d_lag_up = nullptr;
-    void *d_lag_buf = nullptr;
-    double *d_kry = nullptr;         // Krylov basis + b
+    double *d_kry = nullptr;         // Krylov basis + work vectors
     int64_t kry_cols = 0;
 
-    // captured sweeps: [0] isotropic source only, [1] with an angular part
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    const double *graph_ang[2] = {nullptr, nullptr};
+    std::vector<GraphEntry> graphs;
 
     int64_t n_phi() const { return nc * 4 * ng; }
     int64_t n_psi() const { return na * n_phi(); }
@@ -83,6 +97,7 @@ int dalloc(t2_sweeper *s, void **p, size_t bytes)
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
+        (void)hipGetLastError();
         return fail(T2_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) +
                                       " B): " + hipGetErrorString(e));
     }
@@ -100,16 +115,19 @@ int dalloc_t(t2_sweeper *s, P **p, size_t bytes)
     return rc;
 }
 
-void dfree(t2_sweeper *s, void *p, size_t bytes)
+template <typename P>
+void dfree(t2_sweeper *s, P *&p, size_t bytes)
 {
     if (!p)
         return;
-    auto it = std::find(s->allocs.begin(), s->allocs.end(), p);
+    auto it = std::find(s->allocs.begin(), s->allocs.end(),
+                        static_cast<void *>(p));
     if (it != s->allocs.end()) {
         s->allocs.erase(it);
         s->bytes -= bytes;
     }
-    (void)hipFree(p);
+    (void)hipFree(static_cast<void *>(p));
+    p = nullptr;
 }
 
 int set_device(const t2_sweeper *s)
@@ -118,33 +136,30 @@ int set_device(const t2_sweeper *s)
     return T2_OK;
 }
 
+int check(const t2_sweeper *s)
+{
+    if (!s)
+        return fail(T2_ERR_ARG, "null sweeper handle");
+    return set_device(s);
+}
+
+template <typename P>
+int upload(P *dst, const P *src, size_t count)
+{
+    T2_HIP(hipMemcpy(dst, src, sizeof(P) * count, hipMemcpyHostToDevice));
+    return T2_OK;
+}
+
 void drop_graphs(t2_sweeper *s)
 {
-    for (int i = 0; i < 2; i++) {
-        if (s->graph[i])
-            (void)hipGraphExecDestroy(s->graph[i]);
-        s->graph[i] = nullptr;
-        s->graph_ang[i] = nullptr;
-    }
+    for (auto &g : s->graphs)
+        (void)hipGraphExecDestroy(g.exec);
+    s->graphs.clear();
 }
 
+// ------------------------------------------------------------------ sweeps
 template <typename T>
-t2::SweepArgs<T> base_args(const t2_sweeper *s, const double *ang)
-{
-    t2::SweepArgs<T> a{};
-    a.nc = s->nc;
-    a.sigma_mode = s->sigma_mode;
-    a.sigma = static_cast<const T *>(s->d_sigt);
-    a.src_iso = static_cast<const T *>(s->d_src);
-    a.src_ang = ang;
-    a.psi = static_cast<T *>(s->d_psi);
-    a.lag_buf = static_cast<const T *>(s->d_lag_buf);
-    return a;
-}
-
-// Enqueue one whole sweep (lag snapshot + every step) on the stream.
-template <typename T>
-int enqueue_sweep(t2_sweeper *s, const double *ang)
+int enqueue_sweep(t2_sweeper *s, const double *ang, const void *ang_t)
 {
     const int64_t n_lag = int64_t(s->sched.lagged.size());
     if (n_lag > 0)
@@ -152,7 +167,15 @@ int enqueue_sweep(t2_sweeper *s, const double *ang)
             s->d_lag_key, s->d_lag_up, n_lag, s->nc, int(s->ng), s->d_nbr,
             static_cast<const T *>(s->d_psi), static_cast<T *>(s->d_lag_buf),
             s->stream));
-    t2::SweepArgs<T> a = base_args<T>(s, ang);
+    t2::SweepArgs<T> a{};
+    a.nc = s->nc;
+    a.sigma_mode = s->sigma_mode;
+    a.sigma = static_cast<const T *>(s->d_sigt);
+    a.src_iso = static_cast<const T *>(s->d_src);
+    a.src_ang = ang;
+    a.src_ang_t = static_cast<const T *>(ang_t);
+    a.psi = static_cast<T *>(s->d_psi);
+    a.lag_buf = static_cast<const T *>(s->d_lag_buf);
     const auto *geo = static_cast<const t2::ItemGeo<T> *>(s->d_item_geo);
     for (int64_t st = 0; st < s->sched.n_steps; st++) {
         const int64_t off = s->sched.step_off[st];
@@ -165,27 +188,27 @@ int enqueue_sweep(t2_sweeper *s, const double *ang)
     return T2_OK;
 }
 
-int enqueue_sweep_any(t2_sweeper *s, const double *ang)
+int enqueue_sweep_any(t2_sweeper *s, const double *ang, const void *ang_t)
 {
-    return s->precision == T2_FP64 ? enqueue_sweep<double>(s, ang)
-                                   : enqueue_sweep<float>(s, ang);
+    return s->precision == T2_FP64 ? enqueue_sweep<double>(s, ang, ang_t)
+                                   : enqueue_sweep<float>(s, ang, ang_t);
 }
 
-// psi = L^-1 (src_iso + ang); captured once into a hipGraph and replayed.
-int run_sweep(t2_sweeper *s, const double *ang)
+// psi = L^-1 (src + ang + ang_t): every step of the schedule, captured once
+// per source combination into a hipGraph and replayed.
+int run_sweep(t2_sweeper *s, const double *ang, const void *ang_t = nullptr)
 {
     if (!s->use_graph)
-        return enqueue_sweep_any(s, ang);
-    const int slot = ang ? 1 : 0;
-    if (s->graph[slot] && s->graph_ang[slot] != ang) {
-        (void)hipGraphExecDestroy(s->graph[slot]);
-        s->graph[slot] = nullptr;
-    }
-    if (!s->graph[slot]) {
+        return enqueue_sweep_any(s, ang, ang_t);
+    hipGraphExec_t exec = nullptr;
+    for (auto &g : s->graphs)
+        if (g.ang == ang && g.ang_t == ang_t)
+            exec = g.exec;
+    if (!exec) {
         hipGraph_t g = nullptr;
         T2_HIP(hipStreamBeginCapture(s->stream,
                                      hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_sweep_any(s, ang);
+        int rc = enqueue_sweep_any(s, ang, ang_t);
         hipError_t e = hipStreamEndCapture(s->stream, &g);
         if (rc != T2_OK) {
             if (g)
@@ -193,17 +216,16 @@ int run_sweep(t2_sweeper *s, const double *ang)
             return rc;
         }
         T2_HIP(e);
-        hipGraphExec_t ex = nullptr;
-        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         T2_HIP(e);
-        s->graph[slot] = ex;
-        s->graph_ang[slot] = ang;
+        s->graphs.push_back({ang, ang_t, exec});
     }
-    T2_HIP(hipGraphLaunch(s->graph[slot], s->stream));
+    T2_HIP(hipGraphLaunch(exec, s->stream));
     return T2_OK;
 }
 
+// src (T) = q + sigma_s / (4 pi) phi   (either may be null)
 int total_source(t2_sweeper *s, const double *q, const double *phi)
 {
     if (s->precision == T2_FP64)
@@ -255,22 +277,9 @@ int dot(t2_sweeper *s, const double *x, const double *y, int64_t n,
     return fetch(s, out, s->d_scal, 1);
 }
 
-int check(const t2_sweeper *s)
-{
-    if (!s)
-        return fail(T2_ERR_ARG, "null sweeper handle");
-    return set_device(s);
-}
-
-template <typename P>
-int upload(P *dst, const P *src, size_t count)
-{
-    T2_HIP(hipMemcpy(dst, src, sizeof(P) * count, hipMemcpyHostToDevice));
-    return T2_OK;
-}
-
-// sigma_t layout on the device: per group when every cell agrees, per cell
-// (inside the item records) when every group agrees, else [cell][group].
+// ------------------------------------------------------------------ set-up
+// sigma_t layout on the device: per cell (inside the item records) when
+// every group agrees, per group when every cell agrees, else [cell][group].
 int pick_sigma_mode(int64_t nc, int64_t ng, const double *sig)
 {
     bool by_group = true, by_cell = true;
@@ -286,7 +295,8 @@ int pick_sigma_mode(int64_t nc, int64_t ng, const double *sig)
 
 template <typename T>
 int build_items(t2_sweeper *s, const double *d_geo, const int4 *d_adj,
-                const uint32_t *d_work, const double *d_sig_cell)
+                const uint32_t *d_work, const double *d_sig_cell,
+                t2::ItemGeo<T> *out_geo)
 {
     const int64_t n = s->nc * s->na;
     int *d_err = nullptr;
@@ -295,8 +305,7 @@ int build_items(t2_sweeper *s, const double *d_geo, const int4 *d_adj,
     T2_HIP(t2::launch_build_items<T>(
         d_work, n, s->sched.cell_bits, s->nc, s->d_omega, d_geo, d_adj,
         s->d_nbr, d_sig_cell, s->d_lag_key, int64_t(s->sched.lagged.size()),
-        static_cast<t2::ItemGeo<T> *>(s->d_item_geo), s->d_item_topo, d_err,
-        s->stream));
+        out_geo, s->d_item_topo, d_err, s->stream));
     int err = 0;
     T2_HIP(hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost,
                           s->stream));
@@ -307,71 +316,48 @@ int build_items(t2_sweeper *s, const double *d_geo, const int4 *d_adj,
     return T2_OK;
 }
 
-// Device set-up of a new handle: cross sections, item records in step
-// order, and the flux arrays.
-int setup_device(t2_sweeper *s, const t2::HostMesh &hm, const double *omega,
-                 const double *weights, const double *sigma_t,
-                 const double *sigma_s)
+size_t geo_record_size(const t2_sweeper *s)
+{
+    return s->precision == T2_FP64 ? sizeof(t2::ItemGeo<double>)
+                                   : sizeof(t2::ItemGeo<float>);
+}
+
+void free_schedule(t2_sweeper *s)
+{
+    const size_t n_items = size_t(s->nc) * s->na;
+    const int64_t n_lag = int64_t(s->sched.lagged.size());
+    drop_graphs(s);
+    dfree(s, s->d_item_geo, geo_record_size(s) * n_items);
+    dfree(s, s->d_item_geo64, sizeof(t2::ItemGeo<double>) * n_items);
+    dfree(s, s->d_item_topo, sizeof(t2::ItemTopo) * n_items);
+    dfree(s, s->d_lag_key, sizeof(int64_t) * n_lag);
+    dfree(s, s->d_lag_up, sizeof(int32_t) * n_lag);
+    dfree(s, s->d_lag_buf, s->tsize() * size_t(n_lag) * 4 * s->ng);
+}
+
+// Level sets, lagged faces and item records for the current batch count.
+int build_schedule_device(t2_sweeper *s)
 {
     const int64_t nc = s->nc, na = s->na, ng = s->ng;
-    const size_t ts = s->tsize();
-    const int64_t n_phi = nc * 4 * ng;
-    if (n_phi >= (int64_t(1) << 31))
-        return fail(T2_ERR_ARG, "cells x 4 x groups must stay below 2^31");
-    T2_TRY(dalloc_t(s, &s->d_omega, sizeof(double) * 3 * na));
-    T2_TRY(dalloc_t(s, &s->d_wt, sizeof(double) * na));
-    T2_TRY(dalloc_t(s, &s->d_sigs, sizeof(double) * nc * ng));
-    T2_TRY(upload(s->d_omega, omega, 3 * na));
-    T2_TRY(upload(s->d_wt, weights, na));
-    T2_TRY(upload(s->d_sigs, sigma_s, nc * ng));
-
-    s->sigma_mode = pick_sigma_mode(nc, ng, sigma_t);
-    const int64_t n_sig = s->sigma_mode == t2::kSigmaGroup       ? ng
-                          : s->sigma_mode == t2::kSigmaCellGroup ? nc * ng
-                                                                 : 1;
-    std::vector<char> sig(size_t(n_sig) * ts);
-    for (int64_t i = 0; i < n_sig; i++) {
-        const double v = s->sigma_mode == t2::kSigmaCell ? 0.0 : sigma_t[i];
-        if (s->precision == T2_FP64)
-            reinterpret_cast<double *>(sig.data())[i] = v;
-        else
-            reinterpret_cast<float *>(sig.data())[i] = float(v);
-    }
-    T2_TRY(dalloc(s, &s->d_sigt, sig.size()));
-    T2_HIP(hipMemcpy(s->d_sigt, sig.data(), sig.size(),
-                     hipMemcpyHostToDevice));
-
-    // set-up only: per-cell geometry, adjacency, work list
+    free_schedule(s);
+    T2_TRY(t2::build_schedule(s->hm, na, s->omega_h.data(), s->batches,
+                              s->sched));
+    const size_t n_items = size_t(nc) * na;
     std::vector<double> geo(size_t(nc) * t2::kGeoStride, 0.0);
     std::vector<int4> adj(nc);
     std::vector<double> sig_cell(nc);
     for (int64_t c = 0; c < nc; c++) {
         double *g = &geo[size_t(c) * t2::kGeoStride];
         for (int i = 0; i < 12; i++)
-            g[t2::kGeoNormal + i] = hm.normal[12 * c + i];
+            g[t2::kGeoNormal + i] = s->hm.normal[12 * c + i];
         for (int k = 0; k < 4; k++)
-            g[t2::kGeoArea + k] = hm.area[4 * c + k];
-        g[t2::kGeoVolume] = hm.volume[c];
-        adj[c] = make_int4(int(hm.adj[4 * c]), int(hm.adj[4 * c + 1]),
-                           int(hm.adj[4 * c + 2]), int(hm.adj[4 * c + 3]));
-        sig_cell[c] = sigma_t[c * ng];
+            g[t2::kGeoArea + k] = s->hm.area[4 * c + k];
+        g[t2::kGeoVolume] = s->hm.volume[c];
+        adj[c] = make_int4(int(s->hm.adj[4 * c]), int(s->hm.adj[4 * c + 1]),
+                           int(s->hm.adj[4 * c + 2]),
+                           int(s->hm.adj[4 * c + 3]));
+        sig_cell[c] = s->sigma_t_h[c * ng];
     }
-    double *d_geo = nullptr, *d_sig_cell = nullptr;
-    int4 *d_adj = nullptr;
-    uint32_t *d_work = nullptr;
-    const size_t n_items = size_t(nc) * na;
-    T2_TRY(dalloc_t(s, &s->d_nbr, sizeof(uint32_t) * nc));
-    T2_TRY(upload(s->d_nbr, hm.nbr_pack.data(), nc));
-    T2_TRY(dalloc_t(s, &d_geo, sizeof(double) * geo.size()));
-    T2_TRY(dalloc_t(s, &d_adj, sizeof(int4) * nc));
-    T2_TRY(dalloc_t(s, &d_work, sizeof(uint32_t) * n_items));
-    T2_TRY(dalloc_t(s, &d_sig_cell, sizeof(double) * nc));
-    T2_TRY(upload(d_geo, geo.data(), geo.size()));
-    T2_TRY(upload(d_adj, adj.data(), nc));
-    T2_TRY(upload(d_work, s->sched.work.data(), n_items));
-    T2_TRY(upload(d_sig_cell, sig_cell.data(), nc));
-    std::vector<uint32_t>().swap(s->sched.work);
-
     const int64_t n_lag = int64_t(s->sched.lagged.size());
     if (n_lag > 0) {
         std::vector<int64_t> key(n_lag);
@@ -383,23 +369,93 @@ int setup_device(t2_sweeper *s, const t2::HostMesh &hm, const double *omega,
         }
         T2_TRY(dalloc_t(s, &s->d_lag_key, sizeof(int64_t) * n_lag));
         T2_TRY(dalloc_t(s, &s->d_lag_up, sizeof(int32_t) * n_lag));
-        T2_TRY(dalloc(s, &s->d_lag_buf, ts * size_t(n_lag) * 4 * ng));
+        T2_TRY(dalloc(s, &s->d_lag_buf, s->tsize() * size_t(n_lag) * 4 * ng));
         T2_TRY(upload(s->d_lag_key, key.data(), n_lag));
         T2_TRY(upload(s->d_lag_up, up.data(), n_lag));
     }
+    double *d_geo = nullptr, *d_sig_cell = nullptr;
+    int4 *d_adj = nullptr;
+    uint32_t *d_work = nullptr;
+    int rc = T2_OK;
+    auto cleanup = [&]() {
+        dfree(s, d_geo, sizeof(double) * geo.size());
+        dfree(s, d_adj, sizeof(int4) * nc);
+        dfree(s, d_work, sizeof(uint32_t) * n_items);
+        dfree(s, d_sig_cell, sizeof(double) * nc);
+    };
+    if ((rc = dalloc_t(s, &d_geo, sizeof(double) * geo.size())) ||
+        (rc = dalloc_t(s, &d_adj, sizeof(int4) * nc)) ||
+        (rc = dalloc_t(s, &d_work, sizeof(uint32_t) * n_items)) ||
+        (rc = dalloc_t(s, &d_sig_cell, sizeof(double) * nc)) ||
+        (rc = upload(d_geo, geo.data(), geo.size())) ||
+        (rc = upload(d_adj, adj.data(), nc)) ||
+        (rc = upload(d_work, s->sched.work.data(), n_items)) ||
+        (rc = upload(d_sig_cell, sig_cell.data(), nc)) ||
+        (rc = dalloc(s, &s->d_item_geo, geo_record_size(s) * n_items)) ||
+        (rc = dalloc_t(s, &s->d_item_topo, sizeof(t2::ItemTopo) * n_items))) {
+        cleanup();
+        return rc;
+    }
+    std::vector<uint32_t>().swap(s->sched.work);
+    if (s->precision == T2_FP64) {
+        rc = build_items<double>(
+            s, d_geo, d_adj, d_work, d_sig_cell,
+            static_cast<t2::ItemGeo<double> *>(s->d_item_geo));
+    } else {
+        rc = build_items<float>(
+            s, d_geo, d_adj, d_work, d_sig_cell,
+            static_cast<t2::ItemGeo<float> *>(s->d_item_geo));
+        // FP64 records of the same items, for the FP64 defect
+        if (rc == T2_OK)
+            rc = dalloc_t(s, &s->d_item_geo64,
+                          sizeof(t2::ItemGeo<double>) * n_items);
+        if (rc == T2_OK)
+            rc = build_items<double>(s, d_geo, d_adj, d_work, d_sig_cell,
+                                     s->d_item_geo64);
+    }
+    cleanup();
+    return rc;
+}
 
-    const size_t geo_rec = s->precision == T2_FP64
-                               ? sizeof(t2::ItemGeo<double>)
-                               : sizeof(t2::ItemGeo<float>);
-    T2_TRY(dalloc(s, &s->d_item_geo, geo_rec * n_items));
-    T2_TRY(dalloc_t(s, &s->d_item_topo, sizeof(t2::ItemTopo) * n_items));
-    T2_TRY(s->precision == T2_FP64
-               ? build_items<double>(s, d_geo, d_adj, d_work, d_sig_cell)
-               : build_items<float>(s, d_geo, d_adj, d_work, d_sig_cell));
-    dfree(s, d_geo, sizeof(double) * geo.size());
-    dfree(s, d_adj, sizeof(int4) * nc);
-    dfree(s, d_work, sizeof(uint32_t) * n_items);
-    dfree(s, d_sig_cell, sizeof(double) * nc);
+int setup_device(t2_sweeper *s, const double *weights, const double *sigma_s)
+{
+    const int64_t nc = s->nc, na = s->na, ng = s->ng;
+    const size_t ts = s->tsize();
+    const int64_t n_phi = nc * 4 * ng;
+    if (n_phi >= (int64_t(1) << 31))
+        return fail(T2_ERR_ARG, "cells x 4 x groups must stay below 2^31");
+    T2_TRY(dalloc_t(s, &s->d_omega, sizeof(double) * 3 * na));
+    T2_TRY(dalloc_t(s, &s->d_wt, sizeof(double) * na));
+    T2_TRY(dalloc_t(s, &s->d_sigs, sizeof(double) * nc * ng));
+    T2_TRY(upload(s->d_omega, s->omega_h.data(), 3 * na));
+    T2_TRY(upload(s->d_wt, weights, na));
+    T2_TRY(upload(s->d_sigs, sigma_s, nc * ng));
+
+    const double *sigma_t = s->sigma_t_h.data();
+    s->sigma_mode = pick_sigma_mode(nc, ng, sigma_t);
+    const int64_t n_sig = s->sigma_mode == t2::kSigmaGroup       ? ng
+                          : s->sigma_mode == t2::kSigmaCellGroup ? nc * ng
+                                                                 : 1;
+    std::vector<char> sig(size_t(n_sig) * ts);
+    std::vector<double> sig64(n_sig);
+    for (int64_t i = 0; i < n_sig; i++) {
+        const double v = s->sigma_mode == t2::kSigmaCell ? 0.0 : sigma_t[i];
+        sig64[i] = v;
+        if (s->precision == T2_FP64)
+            reinterpret_cast<double *>(sig.data())[i] = v;
+        else
+            reinterpret_cast<float *>(sig.data())[i] = float(v);
+    }
+    T2_TRY(dalloc(s, &s->d_sigt, sig.size()));
+    T2_HIP(hipMemcpy(s->d_sigt, sig.data(), sig.size(),
+                     hipMemcpyHostToDevice));
+    if (s->precision == T2_FP32) {
+        T2_TRY(dalloc_t(s, &s->d_sigt64, sizeof(double) * n_sig));
+        T2_TRY(upload(s->d_sigt64, sig64.data(), n_sig));
+    }
+    T2_TRY(dalloc_t(s, &s->d_nbr, sizeof(uint32_t) * nc));
+    T2_TRY(upload(s->d_nbr, s->hm.nbr_pack.data(), nc));
+    T2_TRY(build_schedule_device(s));
 
     T2_TRY(dalloc(s, &s->d_psi, ts * size_t(na) * n_phi));
     T2_TRY(dalloc(s, &s->d_src, ts * size_t(n_phi)));
@@ -411,6 +467,139 @@ int setup_device(t2_sweeper *s, const t2::HostMesh &hm, const double *omega,
     T2_TRY(total_source(s, nullptr, nullptr));
     T2_HIP(t2::launch_fill(s->d_phi, 0.0, n_phi, s->stream));
     T2_HIP(hipStreamSynchronize(s->stream));
+    return T2_OK;
+}
+
+// ------------------------------------------------------------------ GMRES
+// Krylov storage: basis V[0..m] and `extra` work vectors after it.
+int krylov_space(t2_sweeper *s, int64_t m, int extra, double **V,
+                 double **work)
+{
+    const int64_t n = s->n_phi();
+    const int64_t cols = m + 1 + extra;
+    if (s->kry_cols < cols) {
+        dfree(s, s->d_kry, sizeof(double) * n * s->kry_cols);
+        s->kry_cols = 0;
+        T2_TRY(dalloc_t(s, &s->d_kry, sizeof(double) * n * cols));
+        s->kry_cols = cols;
+    }
+    *V = s->d_kry;
+    *work = s->d_kry + n * (m + 1);
+    return T2_OK;
+}
+
+// y = (I - D L^-1 M S) v: the operator of the scalar-flux equations
+// (src/SourceIteration.cc:76-114), one sweep with no fixed source.
+int transport_apply(t2_sweeper *s, const double *v, double *y)
+{
+    T2_TRY(total_source(s, nullptr, v));
+    T2_TRY(run_sweep(s, nullptr));
+    T2_TRY(psi_to_phi(s, y));
+    T2_HIP(t2::launch_axpby(1.0, v, -1.0, y, s->n_phi(), s->stream));
+    return T2_OK;
+}
+
+// Restarted GMRES(m) from x = 0: classical Gram-Schmidt applied twice and
+// Givens rotations, all in FP64 on the device; stops at rtol * ||b|| or after
+// iter_max matrix applications.  The reference delegates this to PETSc's
+// KSPGMRES (src/KrylovSolver.cc:79-169).
+int gmres(t2_sweeper *s, const double *b, double *x, int64_t m, double rtol,
+          int64_t iter_max, double *V, int64_t *n_iter, double *hist,
+          double *rel_res)
+{
+    const int64_t n = s->n_phi();
+    double bb = 0.0;
+    T2_TRY(dot(s, b, b, n, &bb));
+    const double bnorm = std::sqrt(bb), target = rtol * bnorm;
+    T2_HIP(t2::launch_fill(x, 0.0, n, s->stream));
+    std::vector<double> H(size_t(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1),
+        y(m), h(m + 1), h2(m + 1);
+    int64_t it = 0;
+    double rnorm = bnorm;
+    while (it < iter_max && rnorm > target) {
+        T2_TRY(transport_apply(s, x, V));
+        T2_HIP(t2::launch_axpby(1.0, b, -1.0, V, n, s->stream));  // b - A x
+        double beta2 = 0.0;
+        T2_TRY(dot(s, V, V, n, &beta2));
+        const double beta = std::sqrt(beta2);
+        rnorm = beta;
+        if (beta <= target || beta == 0.0)
+            break;
+        T2_HIP(t2::launch_scale(V, 1.0 / beta, n, s->stream));
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int64_t j = 0;
+        for (; j < m && it < iter_max; j++) {
+            double *w = V + n * (j + 1);
+            T2_TRY(transport_apply(s, V + n * j, w));
+            for (int pass = 0; pass < 2; pass++) {
+                T2_HIP(t2::launch_multi_dot(V, n, int(j + 1), w, n, s->d_red,
+                                            s->d_scal, s->stream));
+                T2_HIP(t2::launch_multi_axpy(V, n, int(j + 1), s->d_scal, w,
+                                             n, s->stream));
+                T2_TRY(fetch(s, pass ? h2.data() : h.data(), s->d_scal,
+                             j + 1));
+            }
+            for (int64_t i = 0; i <= j; i++)
+                H[i * m + j] = h[i] + h2[i];
+            double ww = 0.0;
+            T2_TRY(dot(s, w, w, n, &ww));
+            const double hn = std::sqrt(ww);
+            H[(j + 1) * m + j] = hn;
+            if (hn != 0.0)
+                T2_HIP(t2::launch_scale(w, 1.0 / hn, n, s->stream));
+            for (int64_t i = 0; i < j; i++) {
+                const double t = cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
+                H[(i + 1) * m + j] =
+                    -sn[i] * H[i * m + j] + cs[i] * H[(i + 1) * m + j];
+                H[i * m + j] = t;
+            }
+            const double hjj = H[j * m + j], hj1 = H[(j + 1) * m + j];
+            const double r = std::sqrt(hjj * hjj + hj1 * hj1);
+            cs[j] = hjj / r;
+            sn[j] = hj1 / r;
+            H[j * m + j] = r;
+            H[(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            rnorm = std::fabs(g[j + 1]);
+            if (hist)
+                hist[it] = rnorm / (bnorm > 0 ? bnorm : 1.0);
+            it++;
+            if (rnorm <= target) {
+                j++;
+                break;
+            }
+        }
+        for (int64_t i = j - 1; i >= 0; i--) {
+            double acc = g[i];
+            for (int64_t k = i + 1; k < j; k++)
+                acc -= H[i * m + k] * y[k];
+            y[i] = acc / H[i * m + i];
+        }
+        for (int64_t i = 0; i < j; i++)
+            y[i] = -y[i];
+        T2_HIP(hipMemcpyAsync(s->d_scal, y.data(), sizeof(double) * j,
+                              hipMemcpyHostToDevice, s->stream));
+        T2_HIP(t2::launch_multi_axpy(V, n, int(j), s->d_scal, x, n,
+                                     s->stream));
+        T2_HIP(hipStreamSynchronize(s->stream));
+    }
+    if (n_iter)
+        *n_iter = it;
+    if (rel_res)
+        *rel_res = rnorm / (bnorm > 0 ? bnorm : 1.0);
+    return T2_OK;
+}
+
+int max_abs(t2_sweeper *s, const double *x, double *out)
+{
+    std::vector<double> h(s->n_phi());
+    T2_TRY(fetch(s, h.data(), x, s->n_phi()));
+    double mx = 0.0;
+    for (double v : h)
+        mx = std::max(mx, std::fabs(v));
+    *out = mx;
     return T2_OK;
 }
 
@@ -443,43 +632,44 @@ int t2_sweeper_create(t2_sweeper **out, int device, int64_t n_cells,
     if (gauss != T2_GAUSS_NOPIVOT)
         return fail(T2_ERR_ARG, "only GaussElim NoPivot is implemented");
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
         return fail(T2_ERR_NODEV, "no HIP device");
+    }
     if (device < 0 || device >= ndev)
         return fail(T2_ERR_ARG, "device index out of range");
 
-    t2::HostMesh hm;
-    T2_TRY(t2::build_host_mesh(n_cells, n_nodes, coords, cell_nodes, adj_cell,
-                               f2c, hm));
     auto *s = new (std::nothrow) t2_sweeper();
     if (!s)
         return fail(T2_ERR_NOMEM, "host allocation");
+    int rc = t2::build_host_mesh(n_cells, n_nodes, coords, cell_nodes,
+                                 adj_cell, f2c, s->hm);
+    if (rc) {
+        delete s;
+        return rc;
+    }
     s->device = device;
     s->nc = n_cells;
     s->na = n_angles;
     s->ng = n_groups;
     s->precision = precision;
     s->gauss = gauss;
+    s->omega_h.assign(omega, omega + 3 * n_angles);
+    s->sigma_t_h.assign(sigma_t, sigma_t + n_cells * n_groups);
     if (const char *e = std::getenv("T2_NO_GRAPH"))
         s->use_graph = !(e[0] == '1');
-    int rc = t2::build_schedule(hm, n_angles, omega, s->sched);
-    if (rc) {
-        delete s;
-        return rc;
-    }
-    auto bail = [&](int code) {
-        t2_sweeper_destroy(s);
-        return code;
-    };
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) !=
             hipSuccess) {
         delete s;
         return fail(T2_ERR_HIP, "stream creation failed");
     }
-    rc = setup_device(s, hm, omega, weights, sigma_t, sigma_s);
-    if (rc != T2_OK)
-        return bail(rc);
+    rc = setup_device(s, weights, sigma_s);
+    if (rc != T2_OK) {
+        std::string msg = t2_last_error();
+        t2_sweeper_destroy(s);
+        return fail(rc, msg);
+    }
     *out = s;
     return T2_OK;
 }
@@ -544,6 +734,19 @@ int t2_set_launch(t2_sweeper *s, int items_per_block, int groups_per_lane)
     return T2_OK;
 }
 
+int t2_set_schedule(t2_sweeper *s, int angle_batches)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (angle_batches < 1 || angle_batches > s->na)
+        return fail(T2_ERR_ARG, "angle_batches must be in [1, n_angles]");
+    if (angle_batches == s->batches)
+        return T2_OK;
+    T2_HIP(hipStreamSynchronize(s->stream));
+    s->batches = angle_batches;
+    return build_schedule_device(s);
+}
+
 int t2_sweep_host(t2_sweeper *s, double *psi, const double *source,
                   int zero_psi_bound)
 {
@@ -564,16 +767,14 @@ int t2_sweep_host(t2_sweeper *s, double *psi, const double *source,
                                s->stream));
     if (!s->sched.lagged.empty()) {
         // lagged faces read the caller's previous psi
-        T2_HIP(hipMemcpyAsync(s->d_stage, psi, sizeof(double) * n,
-                              hipMemcpyHostToDevice, s->stream));
-        if (s->precision == T2_FP64) {
-            T2_HIP(t2::launch_ca_to_ac(s->d_stage,
-                                       static_cast<double *>(s->d_psi), s->nc,
-                                       s->na, vg, s->stream));
-        } else {
+        if (s->precision != T2_FP64)
             return fail(T2_ERR_STATE,
                         "host sweep with lagged faces needs FP64 storage");
-        }
+        T2_HIP(hipMemcpyAsync(s->d_stage, psi, sizeof(double) * n,
+                              hipMemcpyHostToDevice, s->stream));
+        T2_HIP(t2::launch_ca_to_ac(s->d_stage,
+                                   static_cast<double *>(s->d_psi), s->nc,
+                                   s->na, vg, s->stream));
     }
     T2_TRY(total_source(s, nullptr, nullptr));
     T2_TRY(run_sweep(s, s->d_hostang));
@@ -594,20 +795,16 @@ int t2_set_source_isotropic(t2_sweeper *s, const double *q)
     T2_TRY(check(s));
     T2_HIP(hipStreamSynchronize(s->stream));
     if (s->d_qang) {
-        dfree(s, s->d_qang, sizeof(double) * s->n_psi());
-        s->d_qang = nullptr;
         drop_graphs(s);
+        dfree(s, s->d_qang, sizeof(double) * s->n_psi());
     }
     if (!q) {
-        if (s->d_qiso)
-            dfree(s, s->d_qiso, sizeof(double) * s->n_phi());
-        s->d_qiso = nullptr;
+        dfree(s, s->d_qiso, sizeof(double) * s->n_phi());
         return T2_OK;
     }
     if (!s->d_qiso)
         T2_TRY(dalloc_t(s, &s->d_qiso, sizeof(double) * s->n_phi()));
-    T2_HIP(hipMemcpy(s->d_qiso, q, sizeof(double) * s->n_phi(),
-                     hipMemcpyHostToDevice));
+    T2_TRY(upload(s->d_qiso, q, s->n_phi()));
     return T2_OK;
 }
 
@@ -682,119 +879,95 @@ int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,
     T2_TRY(check(s));
     if (restart < 1 || restart > 60 || iter_max < 0)
         return fail(T2_ERR_ARG, "restart must be in [1, 60]");
-    const int64_t n = s->n_phi();
-    const int64_t m = restart;
-    if (s->kry_cols < m + 2) {
-        if (s->d_kry)
-            dfree(s, s->d_kry, sizeof(double) * n * s->kry_cols);
-        s->d_kry = nullptr;
-        s->kry_cols = 0;
-        T2_TRY(dalloc_t(s, &s->d_kry, sizeof(double) * n * (m + 2)));
-        s->kry_cols = m + 2;
-    }
-    double *V = s->d_kry;              // m + 1 basis vectors
-    double *b = s->d_kry + n * (m + 1);
-    double *x = s->d_phi;
+    double *V = nullptr, *work = nullptr;
+    T2_TRY(krylov_space(s, restart, 1, &V, &work));
+    double *b = work;
     T2_TRY(zero_psi(s));
-
     // b = D L^-1 q
     T2_TRY(total_source(s, s->d_qiso, nullptr));
     T2_TRY(run_sweep(s, s->d_qang));
     T2_TRY(psi_to_phi(s, b));
-    double bb = 0.0;
-    T2_TRY(dot(s, b, b, n, &bb));
-    const double bnorm = std::sqrt(bb), target = rtol * bnorm;
-    T2_HIP(t2::launch_fill(x, 0.0, n, s->stream));
-
-    // y = (I - D L^-1 M S) v
-    auto apply = [&](const double *v, double *y) -> int {
-        T2_TRY(total_source(s, nullptr, v));
-        T2_TRY(run_sweep(s, nullptr));
-        T2_TRY(psi_to_phi(s, y));
-        T2_HIP(t2::launch_axpby(1.0, v, -1.0, y, n, s->stream));
-        return T2_OK;
-    };
-
-    std::vector<double> H(size_t(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1),
-        y(m), h(m + 1), h2(m + 1);
-    int64_t it = 0;
-    double rnorm = bnorm;
-    while (it < iter_max && rnorm > target) {
-        T2_TRY(apply(x, V));                                  // V0 = A x
-        T2_HIP(t2::launch_axpby(1.0, b, -1.0, V, n, s->stream));  // b - A x
-        double beta2 = 0.0;
-        T2_TRY(dot(s, V, V, n, &beta2));
-        const double beta = std::sqrt(beta2);
-        rnorm = beta;
-        if (beta <= target || beta == 0.0)
-            break;
-        T2_HIP(t2::launch_scale(V, 1.0 / beta, n, s->stream));
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0] = beta;
-        int64_t j = 0;
-        for (; j < m && it < iter_max; j++) {
-            double *w = V + n * (j + 1);
-            T2_TRY(apply(V + n * j, w));
-            // classical Gram-Schmidt, applied twice, in FP64
-            for (int pass = 0; pass < 2; pass++) {
-                T2_HIP(t2::launch_multi_dot(V, n, int(j + 1), w, n, s->d_red,
-                                            s->d_scal, s->stream));
-                T2_HIP(t2::launch_multi_axpy(V, n, int(j + 1), s->d_scal, w,
-                                             n, s->stream));
-                T2_TRY(fetch(s, pass ? h2.data() : h.data(), s->d_scal,
-                             j + 1));
-            }
-            for (int64_t i = 0; i <= j; i++)
-                H[i * m + j] = h[i] + h2[i];
-            double ww = 0.0;
-            T2_TRY(dot(s, w, w, n, &ww));
-            const double hn = std::sqrt(ww);
-            H[(j + 1) * m + j] = hn;
-            if (hn != 0.0)
-                T2_HIP(t2::launch_scale(w, 1.0 / hn, n, s->stream));
-            for (int64_t i = 0; i < j; i++) {
-                const double t = cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
-                H[(i + 1) * m + j] =
-                    -sn[i] * H[i * m + j] + cs[i] * H[(i + 1) * m + j];
-                H[i * m + j] = t;
-            }
-            const double hjj = H[j * m + j], hj1 = H[(j + 1) * m + j];
-            const double r = std::sqrt(hjj * hjj + hj1 * hj1);
-            cs[j] = hjj / r;
-            sn[j] = hj1 / r;
-            H[j * m + j] = r;
-            H[(j + 1) * m + j] = 0.0;
-            g[j + 1] = -sn[j] * g[j];
-            g[j] = cs[j] * g[j];
-            rnorm = std::fabs(g[j + 1]);
-            if (res_hist)
-                res_hist[it] = rnorm / (bnorm > 0 ? bnorm : 1.0);
-            it++;
-            if (rnorm <= target) {
-                j++;
-                break;
-            }
-        }
-        for (int64_t i = j - 1; i >= 0; i--) {
-            double acc = g[i];
-            for (int64_t k = i + 1; k < j; k++)
-                acc -= H[i * m + k] * y[k];
-            y[i] = acc / H[i * m + i];
-        }
-        for (int64_t i = 0; i < j; i++)
-            y[i] = -y[i];
-        T2_HIP(hipMemcpyAsync(s->d_scal, y.data(), sizeof(double) * j,
-                              hipMemcpyHostToDevice, s->stream));
-        T2_HIP(t2::launch_multi_axpy(V, n, int(j), s->d_scal, x, n,
-                                     s->stream));
-        T2_HIP(hipStreamSynchronize(s->stream));
-    }
-    // psi = L^-1 (q + M S x)
-    T2_TRY(total_source(s, s->d_qiso, x));
+    T2_TRY(gmres(s, b, s->d_phi, restart, rtol, iter_max, V, n_iter,
+                 res_hist, nullptr));
+    // psi = L^-1 (q + M S phi)
+    T2_TRY(total_source(s, s->d_qiso, s->d_phi));
     T2_TRY(run_sweep(s, s->d_qang));
     T2_HIP(hipStreamSynchronize(s->stream));
-    if (n_iter)
-        *n_iter = it;
+    return T2_OK;
+}
+
+int t2_solve_refined(t2_sweeper *s, int64_t iter_max, double rtol_sweep,
+                     double rtol_correction, int64_t restart,
+                     t2_refine_info *info, double *phi1)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (s->precision != T2_FP32)
+        return fail(T2_ERR_STATE, "the refined solve needs an FP32 handle");
+    if (restart < 1 || restart > 60 || iter_max < 0)
+        return fail(T2_ERR_ARG, "restart must be in [1, 60]");
+    const int64_t n = s->n_phi();
+    double *V = nullptr, *work = nullptr;
+    T2_TRY(krylov_space(s, restart, 2, &V, &work));
+    double *b = work, *y = work + n;
+    if (!s->d_res)
+        T2_TRY(dalloc_t(s, &s->d_res, sizeof(float) * s->n_psi()));
+    if (!s->d_src64)
+        T2_TRY(dalloc_t(s, &s->d_src64, sizeof(double) * n));
+    t2_refine_info st{};
+    T2_TRY(zero_psi(s));
+
+    // 1. FP32-sweep GMRES on phi, then psi32 and phi1 = D psi32
+    T2_TRY(total_source(s, s->d_qiso, nullptr));
+    T2_TRY(run_sweep(s, s->d_qang));
+    T2_TRY(psi_to_phi(s, b));
+    T2_TRY(gmres(s, b, s->d_phi, restart, rtol_sweep, iter_max, V,
+                 &st.iters_sweep, nullptr, &st.residual_sweep));
+    T2_TRY(total_source(s, s->d_qiso, s->d_phi));
+    T2_TRY(run_sweep(s, s->d_qang));
+    double *phi_1 = s->d_phi_old;
+    T2_TRY(psi_to_phi(s, phi_1));
+    if (phi1)
+        T2_TRY(fetch(s, phi1, phi_1, n));
+
+    // 2. FP64 defect r = q + M S phi1 - L psi32
+    T2_HIP(t2::launch_total_source<double>(s->d_qiso, s->d_sigs, phi_1,
+                                           s->nc, int(s->ng), s->d_src64,
+                                           s->stream));
+    t2::ResidualArgs<float> ra{};
+    ra.geo = s->d_item_geo64;
+    ra.topo = s->d_item_topo;
+    ra.n_items = s->nc * s->na;
+    ra.nc = s->nc;
+    ra.sigma_mode = s->sigma_mode;
+    ra.sigma = s->d_sigt64;
+    ra.src_iso = s->d_src64;
+    ra.src_ang = s->d_qang;
+    ra.lag_up = s->d_lag_up;
+    ra.psi = static_cast<const float *>(s->d_psi);
+    ra.r = s->d_res;
+    T2_HIP(t2::launch_residual<float>(ra, int(s->ng), s->stream));
+
+    // 3. correction: GMRES on (I - D L^-1 M S) D delta = D L^-1 r
+    T2_TRY(total_source(s, nullptr, nullptr));
+    T2_TRY(run_sweep(s, nullptr, s->d_res));
+    T2_TRY(psi_to_phi(s, b));
+    T2_TRY(gmres(s, b, y, restart, rtol_correction, iter_max, V,
+                 &st.iters_correction, nullptr, &st.residual_correction));
+    T2_TRY(total_source(s, nullptr, y));
+    T2_TRY(run_sweep(s, nullptr, s->d_res));
+    T2_TRY(psi_to_phi(s, b));                     // b = D delta
+    double corr = 0.0, base = 0.0;
+    T2_TRY(max_abs(s, b, &corr));
+    T2_HIP(t2::launch_axpby(1.0, phi_1, 1.0, b, n, s->stream));
+    T2_HIP(hipMemcpyAsync(s->d_phi, b, sizeof(double) * n,
+                          hipMemcpyDeviceToDevice, s->stream));
+    T2_TRY(max_abs(s, s->d_phi, &base));
+    st.correction_rel = base > 0 ? corr / base : 0.0;
+    st.sweeps = st.iters_sweep + st.iters_correction + 4 +
+                (st.iters_sweep > 0) + (st.iters_correction > 0);
+    if (info)
+        *info = st;
     return T2_OK;
 }
 

@@ -69,7 +69,12 @@ void angle_levels(const HostMesh &m, const double *om, int32_t angle,
 int build_levels(const HostMesh &m, int64_t na, const double *omega,
                  int32_t *levels, std::vector<LaggedFace> &lagged);
 
+// Angles swept together: octant by octant (sign pattern of Omega), then by
+// index, cut into `batches` nearly equal consecutive groups.
+std::vector<int32_t> angle_batches(int64_t na, const double *omega,
+                                   int batches);
+
 int build_schedule(const HostMesh &m, int64_t na, const double *omega,
-                   Schedule &s);
+                   int batches, Schedule &s);
 
 }  // namespace t2

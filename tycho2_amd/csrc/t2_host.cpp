@@ -269,10 +269,31 @@ int build_levels(const HostMesh &m, int64_t na, const double *omega,
     return T2_OK;
 }
 
-// Work list in step order: step s holds every (angle, cell) at level s of its
-// angle, angle-major and cell-ascending inside the step.
+std::vector<int32_t> angle_batches(int64_t na, const double *omega,
+                                   int batches)
+{
+    std::vector<int32_t> order(na);
+    for (int64_t a = 0; a < na; a++)
+        order[a] = int32_t(a);
+    auto octant = [&](int32_t a) {
+        return (omega[3 * a] > 0) + 2 * (omega[3 * a + 1] > 0) +
+               4 * (omega[3 * a + 2] > 0);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        return octant(x) < octant(y);
+    });
+    const int64_t nb = std::max<int64_t>(1, std::min<int64_t>(batches, na));
+    std::vector<int32_t> batch(na);
+    for (int64_t i = 0; i < na; i++)
+        batch[order[i]] = int32_t(i * nb / na);
+    return batch;
+}
+
+// Work list in step order.  Batch b occupies steps [off_b, off_b + L_b) with
+// L_b the most levels of any of its angles; step off_b + l holds every
+// (angle, cell) of the batch at level l, angle-major and cell-ascending.
 int build_schedule(const HostMesh &m, int64_t na, const double *omega,
-                   Schedule &s)
+                   int batches, Schedule &s)
 {
     const int64_t nc = m.nc;
     int cb = 1;
@@ -290,31 +311,43 @@ int build_schedule(const HostMesh &m, int64_t na, const double *omega,
     } catch (...) {
         return fail(T2_ERR_NOMEM, "host memory for level sets");
     }
+    s.lagged.clear();
     int rc = build_levels(m, na, omega, levels.data(), s.lagged);
     if (rc)
         return rc;
-    int32_t max_lev = 0, min_lev = INT32_MAX;
+    const std::vector<int32_t> batch = angle_batches(na, omega, batches);
+    const int nb = 1 + *std::max_element(batch.begin(), batch.end());
+    std::vector<int32_t> nlev(na), blev(nb, 0);
+    int32_t min_lev = INT32_MAX;
     for (int64_t a = 0; a < na; a++) {
         int32_t mx = 0;
         const int32_t *l = levels.data() + a * nc;
         for (int64_t c = 0; c < nc; c++)
             mx = std::max(mx, l[c]);
-        max_lev = std::max(max_lev, mx + 1);
+        nlev[a] = mx + 1;
         min_lev = std::min(min_lev, mx + 1);
+        blev[batch[a]] = std::max(blev[batch[a]], mx + 1);
     }
-    s.n_steps = max_lev;
+    std::vector<int64_t> boff(nb + 1, 0);
+    for (int b = 0; b < nb; b++)
+        boff[b + 1] = boff[b] + blev[b];
+    s.n_steps = boff[nb];
     s.min_levels = min_lev;
-    s.step_off.assign(max_lev + 1, 0);
-    for (int64_t i = 0; i < na * nc; i++)
-        s.step_off[levels[i] + 1]++;
-    for (int64_t l = 0; l < max_lev; l++)
-        s.step_off[l + 1] += s.step_off[l];
+    s.step_off.assign(s.n_steps + 1, 0);
+    for (int64_t a = 0; a < na; a++) {
+        const int32_t *l = levels.data() + a * nc;
+        for (int64_t c = 0; c < nc; c++)
+            s.step_off[boff[batch[a]] + l[c] + 1]++;
+    }
+    for (int64_t st = 0; st < s.n_steps; st++)
+        s.step_off[st + 1] += s.step_off[st];
     std::vector<int64_t> pos(s.step_off.begin(), s.step_off.end() - 1);
     s.work.resize(size_t(na) * nc);
     for (int64_t a = 0; a < na; a++) {
         const int32_t *l = levels.data() + a * nc;
         for (int64_t c = 0; c < nc; c++)
-            s.work[pos[l[c]]++] = uint32_t(c) | (uint32_t(a) << cb);
+            s.work[pos[boff[batch[a]] + l[c]]++] =
+                uint32_t(c) | (uint32_t(a) << cb);
     }
     return T2_OK;
 }

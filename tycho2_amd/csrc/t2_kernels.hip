@@ -162,7 +162,12 @@ sweep_step_kernel(const SweepArgs<T> p)
     if (item >= p.n_items)
         return;
     const ItemTopo t = p.topo[item];
-    const ItemGeo<T> ge = p.geo[item];
+    // record fields as values (a local copy of the whole record would be
+    // placed in scratch memory by the sigma selection below)
+    const ItemGeo<T> *gp = p.geo + item;
+    const T ge_a12[4] = {gp->a12[0], gp->a12[1], gp->a12[2], gp->a12[3]};
+    const T ge_v20 = gp->v20;
+    const T ge_sig = gp->sig;
     const size_t plane = size_t(p.nc) * CG;
     const T *psi_a = p.psi + size_t(t.angle) * plane;
     const uint32_t own = uint32_t(t.cell) * CG;
@@ -172,18 +177,19 @@ sweep_step_kernel(const SweepArgs<T> p)
     T P = T(0);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        o[k] = ge.a12[k] > T(0) ? ge.a12[k] : T(0);
+        o[k] = ge_a12[k] > T(0) ? ge_a12[k] : T(0);
         P += o[k];
     }
 
 #pragma unroll
     for (int jg = 0; jg < GPL; jg++) {
         const int g = gl + jg * LPI;
-        const T sig = p.sigma_mode == kSigmaGroup ? p.sigma[g]
-                      : p.sigma_mode == kSigmaCell
-                          ? ge.sig
-                          : p.sigma[uint32_t(t.cell) * G + g];
-        const T s = sig * ge.v20;
+        T sig = ge_sig;
+        if (p.sigma_mode != kSigmaCell)
+            sig = p.sigma[p.sigma_mode == kSigmaGroup
+                              ? uint32_t(g)
+                              : uint32_t(t.cell) * G + g];
+        const T s = sig * ge_v20;
 
         // source: V/20 (q_i + sum_j q_j)  (Transport.cc:58-72)
         T q[4];
@@ -196,11 +202,17 @@ sweep_step_kernel(const SweepArgs<T> p)
             for (int v = 0; v < 4; v++)
                 q[v] += T(qa[v * G]);
         }
+        if (p.src_ang_t) {
+            const T *qa = p.src_ang_t + size_t(t.angle) * plane + own + g;
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                q[v] += qa[v * G];
+        }
         const T sumq = (q[0] + q[1]) + (q[2] + q[3]);
         T b[4];
 #pragma unroll
         for (int v = 0; v < 4; v++)
-            b[v] = ge.v20 * (sumq + q[v]);
+            b[v] = ge_v20 * (sumq + q[v]);
 
         // upwind faces: b_i -= a_k (sum_j psi_j + psi_i), i, j != k
         // (Transport.cc:175-271, 740-776); vacuum faces contribute zero
@@ -233,7 +245,7 @@ sweep_step_kernel(const SweepArgs<T> p)
 #pragma unroll
             for (int i = 0; i < 4; i++)
                 if (i != k)
-                    b[i] -= ge.a12[k] * (sum + uv[i]);
+                    b[i] -= ge_a12[k] * (sum + uv[i]);
         }
 
         // M[i][j] = a_i + (1 + d_ij)(s + sum_{k out, k != i, j} a_k)
@@ -243,8 +255,8 @@ sweep_step_kernel(const SweepArgs<T> p)
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++)
-                m[i][j] = (i == j) ? ge.a12[i] + T(2) * (s + (P - o[i]))
-                                   : ge.a12[i] + s + ((P - o[i]) - o[j]);
+                m[i][j] = (i == j) ? ge_a12[i] + T(2) * (s + (P - o[i]))
+                                   : ge_a12[i] + s + ((P - o[i]) - o[j]);
         solve_nopivot<T>(m, b);
 
         T *dst = p.psi + size_t(t.angle) * plane + own + g;
@@ -288,6 +300,91 @@ hipError_t launch_g(const SweepArgs<T> &args, int gpl, int block,
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- residual
+// r_i = V/20 (s_i + sum_j s_j) - sum_{k in} a_k (sum_j u_kj + u_ki)
+//       - sum_j M_ij psi_j   for every (angle, cell, group), in FP64.
+template <typename T, int G>
+__global__ void __launch_bounds__(kMaxBlock)
+residual_kernel(const ResidualArgs<T> p)
+{
+    constexpr uint32_t CG = 4 * G;
+    const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t item = tid / G;
+    const int g = int(tid % G);
+    if (item >= p.n_items)
+        return;
+    const ItemTopo t = p.topo[item];
+    const ItemGeo<double> *gp = p.geo + item;
+    const double ge_a12[4] = {gp->a12[0], gp->a12[1], gp->a12[2], gp->a12[3]};
+    const double ge_v20 = gp->v20;
+    const double ge_sig = gp->sig;
+    const size_t plane = size_t(p.nc) * CG;
+    const T *psi_a = p.psi + size_t(t.angle) * plane;
+    const uint32_t own = uint32_t(t.cell) * CG;
+    double o[4], P = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        o[k] = ge_a12[k] > 0.0 ? ge_a12[k] : 0.0;
+        P += o[k];
+    }
+    double sig = ge_sig;
+    if (p.sigma_mode != kSigmaCell)
+        sig = p.sigma[p.sigma_mode == kSigmaGroup ? uint32_t(g)
+                                                  : uint32_t(t.cell) * G + g];
+    const double s = sig * ge_v20;
+    double q[4], x[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        q[v] = p.src_iso[own + v * G + g];
+        if (p.src_ang)
+            q[v] += p.src_ang[size_t(t.angle) * plane + own + v * G + g];
+        x[v] = double(psi_a[own + v * G + g]);
+    }
+    const double sumq = (q[0] + q[1]) + (q[2] + q[3]);
+    double r[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        r[v] = ge_v20 * (sumq + q[v]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int32_t u = t.up[k];
+        if (u == -1)
+            continue;
+        if (u < -1)
+            u = p.lag_up[-2 - u];
+        const T *pn = psi_a + uint32_t(u) * CG + g;
+        double uv[4];
+        int slot = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (j == k) {
+                uv[j] = 0.0;
+                continue;
+            }
+            const uint32_t vtx = (t.meta >> (6 * k + 2 * slot)) & 3u;
+            uv[j] = double(pn[vtx * G]);
+            slot++;
+        }
+        const double sum = (uv[0] + uv[1]) + (uv[2] + uv[3]);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (i != k)
+                r[i] -= ge_a12[k] * (sum + uv[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const double m = (i == j) ? ge_a12[i] + 2.0 * (s + (P - o[i]))
+                                      : ge_a12[i] + s + ((P - o[i]) - o[j]);
+            r[i] -= m * x[j];
+        }
+    float *dst = p.r + size_t(t.angle) * plane + own + g;
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        dst[v * G] = float(r[v]);
 }
 
 // ---------------------------------------------------------------- lagging
@@ -556,6 +653,34 @@ hipError_t launch_sweep_step(const SweepArgs<T> &args, int groups,
     }
 }
 
+template <typename T, int G>
+hipError_t launch_residual_g(const ResidualArgs<T> &args, hipStream_t stream)
+{
+    if (args.n_items <= 0)
+        return hipSuccess;
+    const int64_t threads = args.n_items * G;
+    const unsigned grid = unsigned((threads + kMaxBlock - 1) / kMaxBlock);
+    hipLaunchKernelGGL((residual_kernel<T, G>), dim3(grid), dim3(kMaxBlock),
+                       0, stream, args);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_residual(const ResidualArgs<T> &args, int groups,
+                           hipStream_t stream)
+{
+    switch (groups) {
+    case 1: return launch_residual_g<T, 1>(args, stream);
+    case 2: return launch_residual_g<T, 2>(args, stream);
+    case 4: return launch_residual_g<T, 4>(args, stream);
+    case 8: return launch_residual_g<T, 8>(args, stream);
+    case 16: return launch_residual_g<T, 16>(args, stream);
+    case 32: return launch_residual_g<T, 32>(args, stream);
+    case 64: return launch_residual_g<T, 64>(args, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 template <typename T>
 hipError_t launch_lag_gather(const int64_t *lag_key, const int32_t *lag_up,
                              int64_t n_lag, int64_t nc, int groups,
@@ -675,6 +800,8 @@ hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t stream)
         hipStream_t);                                                         \
     template hipError_t launch_sweep_step<T>(const SweepArgs<T> &, int, int,  \
                                              int, hipStream_t);               \
+    template hipError_t launch_residual<T>(const ResidualArgs<T> &, int,      \
+                                           hipStream_t);                      \
     template hipError_t launch_lag_gather<T>(const int64_t *, const int32_t *,\
                                              int64_t, int64_t, int,           \
                                              const uint32_t *, const T *,     \

@@ -54,9 +54,32 @@ struct SweepArgs {
     const T *sigma;            // [nc][G] (cell-group) or [G] (group)
     const T *src_iso;          // [nc][4][G] isotropic total source
     const double *src_ang;     // [na][nc][4][G] angle-dependent part or null
+    const T *src_ang_t;        // [na][nc][4][G] same, in T, or null
     T *psi;                    // [na][nc][4][G]
     const T *lag_buf;          // [n_lag][4][G] upwind values held pre-sweep
 };
+
+// Defect of the coupled transport equations for a stored angular flux,
+// r = q + S phi - L psi, evaluated in FP64 from FP64 item records (lagged
+// faces read the upwind cell's current psi: the operator is the full one).
+template <typename T>
+struct ResidualArgs {
+    const ItemGeo<double> *geo;
+    const ItemTopo *topo;
+    int64_t n_items;
+    int64_t nc;
+    int sigma_mode;
+    const double *sigma;       // FP64, layout by sigma_mode
+    const double *src_iso;     // [nc][4][G] FP64 total isotropic source
+    const double *src_ang;     // [na][nc][4][G] FP64 or null
+    const int32_t *lag_up;     // upwind cell of lagged face i
+    const T *psi;
+    float *r;                  // [na][nc][4][G]
+};
+
+template <typename T>
+hipError_t launch_residual(const ResidualArgs<T> &args, int groups,
+                           hipStream_t stream);
 
 // Builds the item records from the step-ordered work list; sets *err to a
 // nonzero value if an item has more than three incoming interior faces.

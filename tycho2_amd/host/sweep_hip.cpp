@@ -1,0 +1,432 @@
+// sweep_hip.x -- C++ host driver of the MI355X sweep engine.
+//
+// The same structure as the reference's executable (src/Main.cc): read a
+// serial mesh and an input deck, build the quadrature and cross sections,
+// construct a sweeper through the SweeperAbstract interface
+// (src/SweeperAbstract.hh:47-70) and call solve().  SweeperHIP::sweep goes
+// through the engine's C ABI (include/tycho2_sweep.h); the host-side source
+// iteration below calls it exactly as the reference's
+// SourceIteration::fixedPoint does (src/SourceIteration.cc:127-207), and
+// "DeviceLoop true" in the deck runs the outer iteration on the device.
+//
+// Usage: sweep_hip.x <mesh.smesh> <input.deck>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tycho2_sweep.h"
+
+namespace {
+
+// ------------------------------------------------------------------ data
+// Angular flux in the reference's layout [cell][angle][vertex][group]
+// (src/PsiData.hh:162); scalar flux [cell][vertex][group] (:369).
+struct PsiData {
+    int64_t nc, na, ng;
+    std::vector<double> v;
+    PsiData(int64_t c, int64_t a, int64_t g) : nc(c), na(a), ng(g),
+        v(size_t(c) * a * 4 * g, 0.0) {}
+    double &operator()(int64_t g, int64_t vx, int64_t a, int64_t c)
+    {
+        return v[((c * na + a) * 4 + vx) * ng + g];
+    }
+};
+
+struct PhiData {
+    int64_t nc, ng;
+    std::vector<double> v;
+    PhiData(int64_t c, int64_t g) : nc(c), ng(g), v(size_t(c) * 4 * g, 0.0) {}
+};
+
+void check(int rc)
+{
+    if (rc != T2_OK)
+        throw std::runtime_error(std::string("engine: ") + t2_last_error());
+}
+
+// ------------------------------------------------------------------ deck
+// key value pairs, '#' comments (the reference's KeyValueReader format)
+struct Deck {
+    std::map<std::string, std::string> kv;
+    explicit Deck(const std::string &path)
+    {
+        std::ifstream in(path);
+        if (!in)
+            throw std::runtime_error("cannot open deck " + path);
+        std::string line;
+        while (std::getline(in, line)) {
+            const size_t hash = line.find('#');
+            if (hash != std::string::npos)
+                line.resize(hash);
+            std::istringstream ls(line);
+            std::string k, v;
+            if (ls >> k >> v)
+                kv[k] = v;
+        }
+    }
+    std::string str(const std::string &k, const std::string &def = "") const
+    {
+        auto it = kv.find(k);
+        if (it == kv.end()) {
+            if (def.empty())
+                throw std::runtime_error("deck key not found: " + k);
+            return def;
+        }
+        return it->second;
+    }
+    double num(const std::string &k) const { return std::stod(str(k)); }
+    bool flag(const std::string &k, const std::string &def = "false") const
+    {
+        return str(k, def) == "true";
+    }
+};
+
+// ------------------------------------------------------------------ mesh
+// .smesh v2 (src/SerialMesh.hh:43-68) and the per-cell connectivity the
+// engine takes (src/TychoMeshIO.cc:153-296): local face k is opposite cell
+// vertex k, face vertex j is the face record's node j.
+struct Mesh {
+    int64_t nc = 0, nf = 0, nn = 0;
+    std::vector<int64_t> cell_faces, cell_nodes, face_cells, face_nodes;
+    std::vector<double> coords;
+    std::vector<int64_t> adj, f2c;
+};
+
+Mesh read_smesh(const std::string &path)
+{
+    std::ifstream in(path, std::ios::binary);
+    if (!in)
+        throw std::runtime_error("cannot open mesh " + path);
+    char name[32];
+    uint64_t hdr[4];
+    in.read(name, 32);
+    in.read(reinterpret_cast<char *>(hdr), sizeof(hdr));
+    if (!in || std::strncmp(name, "Tycho 2 Serial Mesh", 32) != 0 ||
+        hdr[0] != 2)
+        throw std::runtime_error(path + ": not a Tycho 2 serial mesh v2");
+    Mesh m;
+    m.nc = int64_t(hdr[1]);
+    m.nf = int64_t(hdr[2]);
+    m.nn = int64_t(hdr[3]);
+    std::vector<int64_t> cells(size_t(m.nc) * 9), faces(size_t(m.nf) * 5);
+    m.coords.resize(size_t(m.nn) * 3);
+    in.read(reinterpret_cast<char *>(cells.data()), cells.size() * 8);
+    in.read(reinterpret_cast<char *>(faces.data()), faces.size() * 8);
+    in.read(reinterpret_cast<char *>(m.coords.data()), m.coords.size() * 8);
+    if (!in)
+        throw std::runtime_error(path + ": truncated");
+    m.cell_faces.resize(4 * m.nc);
+    m.cell_nodes.resize(4 * m.nc);
+    for (int64_t c = 0; c < m.nc; c++)
+        for (int i = 0; i < 4; i++) {
+            m.cell_faces[4 * c + i] = cells[9 * c + i];
+            m.cell_nodes[4 * c + i] = cells[9 * c + 4 + i];
+        }
+    m.face_cells.resize(2 * m.nf);
+    m.face_nodes.resize(3 * m.nf);
+    for (int64_t f = 0; f < m.nf; f++) {
+        m.face_cells[2 * f] = faces[5 * f];
+        m.face_cells[2 * f + 1] = faces[5 * f + 1];
+        for (int j = 0; j < 3; j++)
+            m.face_nodes[3 * f + j] = faces[5 * f + 2 + j];
+    }
+    m.adj.assign(4 * m.nc, -1);
+    m.f2c.assign(12 * m.nc, -1);
+    for (int64_t c = 0; c < m.nc; c++) {
+        const int64_t *cn = &m.cell_nodes[4 * c];
+        for (int s = 0; s < 4; s++) {
+            const int64_t f = m.cell_faces[4 * c + s];
+            const int64_t *fn = &m.face_nodes[3 * f];
+            int lf = -1;
+            for (int v = 0; v < 4; v++)
+                if (cn[v] != fn[0] && cn[v] != fn[1] && cn[v] != fn[2])
+                    lf = v;
+            const int64_t c0 = m.face_cells[2 * f], c1 = m.face_cells[2 * f + 1];
+            m.adj[4 * c + lf] = c1 < 0 ? -1 : (c0 == c ? c1 : c0);
+            for (int j = 0; j < 3; j++)
+                for (int v = 0; v < 4; v++)
+                    if (fn[j] == cn[v])
+                        m.f2c[(4 * c + lf) * 3 + j] = v;
+        }
+    }
+    return m;
+}
+
+// ------------------------------------------------------------------ quadrature
+// Chebyshev-Legendre product set (src/Quadrature.cc:150-244) with
+// Gauss-Legendre nodes and weights from Newton's method in long double,
+// rounded to 16 decimals as the reference's tables are.
+void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w)
+{
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    for (int i = 0; i < n; i++) {
+        long double r = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+        long double dp = 0;
+        for (int it = 0; it < 100; it++) {
+            long double p0 = 1, p1 = r;
+            for (int k = 2; k <= n; k++) {
+                long double p2 = ((2 * k - 1) * r * p1 - (k - 1) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (r * p1 - p0) / (r * r - 1);
+            long double dx = p1 / dp;
+            r -= dx;
+            if (std::fabs(dx) < 1e-19L)
+                break;
+        }
+        long double p0 = 1, p1 = r;
+        for (int k = 2; k <= n; k++) {
+            long double p2 = ((2 * k - 1) * r * p1 - (k - 1) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = n * (r * p1 - p0) / (r * r - 1);
+        long double wt = 2 / ((1 - r * r) * dp * dp);
+        char buf[64];
+        std::snprintf(buf, sizeof(buf), "%.16Lf", r);
+        x[n - 1 - i] = std::strtod(buf, nullptr);
+        std::snprintf(buf, sizeof(buf), "%.16Lf", wt);
+        w[n - 1 - i] = std::strtod(buf, nullptr);
+    }
+}
+
+void quadrature(int sn, std::vector<double> &omega, std::vector<double> &wt)
+{
+    if (sn < 2 || sn > 30 || sn % 2)
+        throw std::runtime_error("Quadrature order not supported.");
+    std::vector<double> gn, gw;
+    gauss_legendre(sn, gn, gw);
+    const int na = 2 * sn * sn;
+    omega.assign(3 * na, 0.0);
+    wt.assign(na, 0.0);
+    for (int i = 0; i < sn; i++)
+        for (int k = 0; k < 2 * sn; k++) {
+            const double phi = (2.0 * M_PI) / (2 * sn) * (k + 0.5);
+            const int idx = i * 2 * sn + k;
+            omega[3 * idx] = std::sqrt(1.0 - gn[i] * gn[i]) * std::cos(phi);
+            omega[3 * idx + 1] = std::sqrt(1.0 - gn[i] * gn[i]) * std::sin(phi);
+            omega[3 * idx + 2] = gn[i];
+            wt[idx] = gw[i] * (2.0 * M_PI) / (2 * sn);
+        }
+}
+
+// ------------------------------------------------------------------ sweeper
+class SweeperAbstract {
+public:
+    virtual ~SweeperAbstract() = default;
+    virtual void sweep(PsiData &psi, const PsiData &source,
+                       bool zeroPsiBound = false) = 0;
+    virtual void solve() = 0;
+};
+
+struct Problem {
+    Mesh mesh;
+    int sn = 0;
+    int64_t na = 0, ng = 0;
+    std::vector<double> omega, wt, sigma_t, sigma_s;
+};
+
+// The reference's manufactured hat source on [0,100]^3 (src/Problem.cc:47-82)
+void hat_source(const Problem &p, PsiData &src)
+{
+    for (int64_t c = 0; c < p.mesh.nc; c++)
+        for (int64_t a = 0; a < p.na; a++)
+            for (int v = 0; v < 4; v++) {
+                const double *x = &p.mesh.coords[3 * p.mesh.cell_nodes[4 * c + v]];
+                const double dx = x[0] - 50.0, dy = x[1] - 50.0, dz = x[2] - 50.0;
+                const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                double val = 0.0;
+                if (r <= 30.0)
+                    val = -dx / (30.0 * r) * p.omega[3 * a] -
+                          dy / (30.0 * r) * p.omega[3 * a + 1] -
+                          dz / (30.0 * r) * p.omega[3 * a + 2] +
+                          (p.sigma_t[c * p.ng] - p.sigma_s[c * p.ng]) *
+                              (1.0 - r / 30.0);
+                for (int64_t g = 0; g < p.ng; g++)
+                    src(g, v, a, c) = val;
+            }
+}
+
+void psi_to_phi(const Problem &p, const PsiData &psi, PhiData &phi)
+{
+    std::fill(phi.v.begin(), phi.v.end(), 0.0);
+    for (int64_t c = 0; c < p.mesh.nc; c++)
+        for (int64_t a = 0; a < p.na; a++)
+            for (int64_t i = 0; i < 4 * p.ng; i++)
+                phi.v[c * 4 * p.ng + i] +=
+                    psi.v[(c * p.na + a) * 4 * p.ng + i] * p.wt[a];
+}
+
+class SweeperHIP : public SweeperAbstract {
+public:
+    SweeperHIP(const Problem &p, const Deck &deck) : c_p(p), c_deck(deck)
+    {
+        const int prec = deck.str("Precision", "FP64") == "FP32" ? T2_FP32
+                                                                 : T2_FP64;
+        check(t2_sweeper_create(&c_engine, 0, p.mesh.nc, p.mesh.nn,
+                                p.mesh.coords.data(), p.mesh.cell_nodes.data(),
+                                p.mesh.adj.data(), p.mesh.f2c.data(), p.na,
+                                p.omega.data(), p.wt.data(), p.ng,
+                                p.sigma_t.data(), p.sigma_s.data(), prec,
+                                T2_GAUSS_NOPIVOT));
+    }
+    ~SweeperHIP() override { t2_sweeper_destroy(c_engine); }
+
+    void sweep(PsiData &psi, const PsiData &source, bool zeroPsiBound) override
+    {
+        check(t2_sweep_host(c_engine, psi.v.data(), source.v.data(),
+                            zeroPsiBound ? 1 : 0));
+    }
+
+    void solve() override;
+
+    PsiData psi{0, 0, 0};
+    int64_t iterations = 0;
+
+private:
+    void fixed_point(PsiData &psi, const PsiData &source);
+
+    const Problem &c_p;
+    const Deck &c_deck;
+    t2_sweeper *c_engine = nullptr;
+};
+
+// Source iteration through sweep() (src/SourceIteration.cc:127-207):
+// total source = q + sigma_s/(4 pi) phi, sweep, phi = D psi, stop on the
+// L1-relative phi change.
+void SweeperHIP::fixed_point(PsiData &psi, const PsiData &source)
+{
+    const Problem &p = c_p;
+    PsiData total(p.mesh.nc, p.na, p.ng);
+    PhiData phi_new(p.mesh.nc, p.ng), phi_old(p.mesh.nc, p.ng);
+    psi_to_phi(p, psi, phi_new);
+    const int64_t iter_max = int64_t(c_deck.num("iterMax"));
+    const double err_max = c_deck.num("errMax");
+    double err = 1.0;
+    iterations = 0;
+    while (iterations < iter_max && err > err_max) {
+        phi_old.v = phi_new.v;
+        for (int64_t c = 0; c < p.mesh.nc; c++)
+            for (int64_t a = 0; a < p.na; a++)
+                for (int v = 0; v < 4; v++)
+                    for (int64_t g = 0; g < p.ng; g++)
+                        total(g, v, a, c) =
+                            source.v[((c * p.na + a) * 4 + v) * p.ng + g] +
+                            p.sigma_s[c * p.ng + g] / (4.0 * M_PI) *
+                                phi_old.v[(c * 4 + v) * p.ng + g];
+        sweep(psi, total, false);
+        psi_to_phi(p, psi, phi_new);
+        double num = 0.0, den = 0.0;
+        for (size_t i = 0; i < phi_new.v.size(); i++) {
+            num += std::fabs(phi_new.v[i] - phi_old.v[i]);
+            den += std::fabs(phi_new.v[i]);
+        }
+        err = num / den;
+        std::printf("   iteration: %lld   error: %e\n", (long long)iterations,
+                    err);
+        iterations++;
+    }
+}
+
+void SweeperHIP::solve()
+{
+    const Problem &p = c_p;
+    PsiData source(p.mesh.nc, p.na, p.ng);
+    hat_source(p, source);
+    psi = PsiData(p.mesh.nc, p.na, p.ng);
+    if (!c_deck.flag("DeviceLoop")) {
+        if (!c_deck.flag("SourceIteration", "true"))
+            throw std::runtime_error("the host loop runs source iteration; "
+                                     "set DeviceLoop true for Krylov");
+        fixed_point(psi, source);
+        return;
+    }
+    check(t2_set_source_angular(c_engine, source.v.data()));
+    if (c_deck.flag("SourceIteration", "true"))
+        check(t2_solve_source_iteration(c_engine, int64_t(c_deck.num("iterMax")),
+                                        c_deck.num("errMax"), &iterations,
+                                        nullptr));
+    else
+        check(t2_solve_krylov(c_engine, int64_t(c_deck.num("iterMax")),
+                              c_deck.num("errMax"), 30, &iterations, nullptr));
+    check(t2_get_psi(c_engine, psi.v.data()));
+}
+
+// "Tycho 2 Psi Output" v1 (src/PsiData.cc:46-106)
+void write_psi(const std::string &path, const PsiData &psi)
+{
+    std::ofstream out(path, std::ios::binary);
+    char name[32] = {0};
+    std::strncpy(name, "Tycho 2 Psi Output", sizeof(name));
+    uint64_t hdr[4] = {1, uint64_t(psi.nc), uint64_t(psi.na), uint64_t(psi.ng)};
+    out.write(name, 32);
+    out.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+    out.write(reinterpret_cast<const char *>(psi.v.data()), psi.v.size() * 8);
+    if (!out)
+        throw std::runtime_error("cannot write " + path);
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc < 3) {
+        std::printf("Usage: ./sweep_hip.x <serial mesh> <input deck>\n");
+        return 0;
+    }
+    try {
+        Deck deck(argv[2]);
+        const std::string type = deck.str("SweepType");
+        if (type != "HIP")
+            throw std::runtime_error("Sweep type not recognized: " + type +
+                                     " (this executable runs SweepType HIP)");
+        Problem p;
+        p.mesh = read_smesh(argv[1]);
+        p.sn = int(deck.num("snOrder"));
+        p.ng = int64_t(deck.num("nGroups"));
+        quadrature(p.sn, p.omega, p.wt);
+        p.na = int64_t(p.wt.size());
+        // per-material cross sections (src/Problem.cc:149-172)
+        const double st[2] = {deck.num("sigmaT1"), deck.num("sigmaT2")};
+        const double ss[2] = {deck.num("sigmaS1"), deck.num("sigmaS2")};
+        std::vector<int64_t> material(p.mesh.nc, 1);
+        {
+            std::ifstream in(argv[1], std::ios::binary);
+            std::vector<int64_t> cells(size_t(p.mesh.nc) * 9);
+            in.seekg(64);
+            in.read(reinterpret_cast<char *>(cells.data()), cells.size() * 8);
+            for (int64_t c = 0; c < p.mesh.nc; c++)
+                material[c] = cells[9 * c + 8];
+        }
+        p.sigma_t.resize(p.mesh.nc * p.ng);
+        p.sigma_s.resize(p.mesh.nc * p.ng);
+        for (int64_t c = 0; c < p.mesh.nc; c++) {
+            if (material[c] != 1 && material[c] != 2)
+                throw std::runtime_error("Material index out of bounds.");
+            for (int64_t g = 0; g < p.ng; g++) {
+                p.sigma_t[c * p.ng + g] = st[material[c] - 1];
+                p.sigma_s[c * p.ng + g] = ss[material[c] - 1];
+            }
+        }
+        SweeperHIP sweeper(p, deck);
+        SweeperAbstract &abstract = sweeper;
+        abstract.solve();
+        std::printf("iterations: %lld\n", (long long)sweeper.iterations);
+        if (deck.flag("OutputFile"))
+            write_psi(deck.str("OutputFilename"), sweeper.psi);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "sweep_hip.x: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

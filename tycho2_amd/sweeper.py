@@ -43,6 +43,10 @@ SIGNATURES = {
     "t2_sweeper_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "t2_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
                                      ctypes.c_int]),
+    "t2_set_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "t2_solve_refined": (ctypes.c_int, [
+        ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+        ctypes.c_int64, ctypes.c_void_p, _dp]),
     "t2_sweep_host": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp,
                                      ctypes.c_int]),
     "t2_set_source_isotropic": (ctypes.c_int, [ctypes.c_void_p, _dp]),
@@ -72,6 +76,15 @@ class T2Info(ctypes.Structure):
                 ("psi_bytes", ctypes.c_int64),
                 ("device_bytes", ctypes.c_int64),
                 ("precision", ctypes.c_int32), ("gauss", ctypes.c_int32)]
+
+
+class T2RefineInfo(ctypes.Structure):
+    _fields_ = [("iters_sweep", ctypes.c_int64),
+                ("iters_correction", ctypes.c_int64),
+                ("sweeps", ctypes.c_int64),
+                ("residual_sweep", ctypes.c_double),
+                ("residual_correction", ctypes.c_double),
+                ("correction_rel", ctypes.c_double)]
 
 
 class T2Error(RuntimeError):
@@ -204,6 +217,9 @@ class SweeperHIP:
         _check(lib().t2_set_launch(self.handle, items_per_block,
                                    groups_per_lane))
 
+    def set_schedule(self, angle_batches: int) -> None:
+        _check(lib().t2_set_schedule(self.handle, int(angle_batches)))
+
     # -- SweeperAbstract::sweep
     def sweep(self, psi: np.ndarray, source: np.ndarray,
               zero_psi_bound: bool = False) -> np.ndarray:
@@ -244,6 +260,18 @@ class SweeperHIP:
                                      int(restart), ctypes.byref(nit),
                                      _ptr(hist)))
         return hist[:nit.value].copy()
+
+    def solve_refined(self, iter_max=300, rtol_sweep=1e-6,
+                      rtol_correction=1e-5, restart=20, want_phi1=False):
+        """FP32 sweeps + FP64 defect correction (FP32 handles).  Returns
+        (info dict, phi after the FP32 stage or None)."""
+        info = T2RefineInfo()
+        phi1 = np.empty((self.nc, 4, self.ng)) if want_phi1 else None
+        _check(lib().t2_solve_refined(
+            self.handle, int(iter_max), float(rtol_sweep),
+            float(rtol_correction), int(restart), ctypes.byref(info),
+            _ptr(phi1) if want_phi1 else None))
+        return {k: getattr(info, k) for k, _ in T2RefineInfo._fields_}, phi1
 
     def sweep_step(self) -> None:
         _check(lib().t2_sweep_step(self.handle))
