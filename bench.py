@@ -5,20 +5,27 @@ Workload (default): the 1,296,000-tet precision problem -- unit cube,
 product quadrature (128 angles), 16 groups, sigma_t_g = 1 + 0.1 g,
 sigma_s = 0.9 sigma_t, q = 1, vacuum boundaries; synthetic, generated here.
 
-One step = one source-iteration step on the device: total source
-q + sigma_s/(4 pi) phi, one full transport sweep over every (cell, angle,
-group), and phi = D psi.  The timed region starts with all inputs resident in
-HBM; K steps are bracketed by HIP events on the engine's stream.
+Two paths run on the same device, each autotuned first:
+  plain    FP64 sweeps, FP64 psi; GMRES(20) to 1e-10
+  refined  FP32 sweeps, FP32 psi, inside FP64 defect correction
+           (t2_solve_refined); its scalar flux is compared with the plain
+           path's after the FP32 stage and after the correction.
 
-The JSON line also carries
+A step is one source-iteration step on the device (total source, one full
+sweep of every (cell, angle, group), phi = D psi); K steps are bracketed by
+HIP events on the engine's stream with every input resident in HBM.
+`value` is the refined path's solves/s over its whole solve when it meets
+the 1e-10 bar (all of its sweeps' solves / its wall time); the plain path's
+numbers sit beside it.
+
+Also on the line:
   roofline      the sweep kernels (dominant): algorithmic bytes per solve
-                B = w (4 + 3 f_in) + (4 w_src + 32) / N_a with
-                f_in = 2 - 1/n (the mean count of upwind interior faces),
-                divided by their average launch duration measured with HIP
-                events over a run of sweeps, against 8 TB/s HBM;
+                B = w (4 + 3 f_in) + (4 w + 32) / N_a with f_in = 2 - 1/n
+                (mean upwind interior faces), over their average launch
+                duration measured with HIP events, against 8 TB/s HBM;
   cpu_baseline  the CPU oracle (a port of the reference's FP64 sweep) on the
                 host cores, over a bounded sample of the same sweep (the
-                first S angles of the same mesh, all 16 groups).
+                first S angles of the same mesh, all groups).
 """
 from __future__ import annotations
 
@@ -35,65 +42,69 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-class HipEvents:
-    """hipEvent timing on a given stream (the engine's own stream)."""
-
-    def __init__(self):
-        self.hip = ctypes.CDLL("libamdhip64.so")
-
-    def _ok(self, rc):
-        if rc != 0:
-            raise RuntimeError(f"HIP event call failed ({rc})")
-
-    def create(self):
-        e = ctypes.c_void_p()
-        self._ok(self.hip.hipEventCreate(ctypes.byref(e)))
-        return e
-
-    def record(self, e, stream):
-        self._ok(self.hip.hipEventRecord(e, ctypes.c_void_p(stream)))
-
-    def elapsed_ms(self, a, b):
-        self._ok(self.hip.hipEventSynchronize(b))
-        ms = ctypes.c_float()
-        self._ok(self.hip.hipEventElapsedTime(ctypes.byref(ms), a, b))
-        return float(ms.value)
-
-
 def alg_bytes_per_solve(width: int, n: int, na: int) -> float:
     f_in = 2.0 - 1.0 / n
     return width * (4.0 + 3.0 * f_in) + (4.0 * width + 4.0 * 8.0) / na
 
 
-def time_region(ev, s, fn, count):
-    start, stop = ev.create(), ev.create()
-    stream = s.stream()
-    s.synchronize()
-    ev.record(start, stream)
-    for _ in range(count):
-        fn()
-    ev.record(stop, stream)
-    return ev.elapsed_ms(start, stop)
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
 
 
-def gpu_leg(p, precision, steps, warmup, ev):
-    from tycho2_amd.sweeper import SweeperHIP
+def path_leg(p, precision, steps, warmup, ev, tune, solve):
+    """Autotune, time steps and sweeps, and (optionally) solve to 1e-10."""
+    import numpy as np
+    from tycho2_amd import autotune
+    from tycho2_amd.sweeper import FP32, SweeperHIP
+    from tycho2_amd.timing import time_region
     t0 = time.time()
     s = SweeperHIP.from_problem(p, precision)
-    setup_s = time.time() - t0
+    out = dict(setup_s=round(time.time() - t0, 2))
     try:
+        if tune:
+            best, _ = autotune.tune(s, reps=2)
+            out["tuned"] = best
+            log(f"autotune: {best}")
         info = s.info()
         for _ in range(warmup):
             s.sweep_step()
         s.synchronize()
-        ms_steps = time_region(ev, s, s.sweep_step, steps)
-        ms_sweeps = time_region(ev, s, s.sweep_only, steps)
-        s.synchronize()
+        out["ms_per_step"] = time_region(ev, s, s.sweep_step, steps) / steps
+        out["ms_per_sweep"] = time_region(ev, s, s.sweep_only, steps) / steps
+        out["n_steps"] = info["n_steps"]
+        out["psi_gb"] = info["psi_bytes"] / 1e9
+        if solve:
+            cfg = p.solver
+            if precision == FP32:
+                holder = {}
+
+                def run():
+                    holder["info"], holder["phi1"] = s.solve_refined(
+                        cfg["iter_max"], 1e-6, 1e-5, cfg["restart"],
+                        want_phi1=True)
+                ms = time_region(ev, s, run, 1)
+                out["solve_ms"] = ms
+                out["refine"] = holder["info"]
+                out["phi1"] = holder["phi1"]
+                out["sweeps"] = holder["info"]["sweeps"]
+            else:
+                holder = {}
+
+                def run():
+                    holder["hist"] = s.solve_krylov(cfg["iter_max"],
+                                                    cfg["tol"],
+                                                    cfg["restart"])
+                ms = time_region(ev, s, run, 1)
+                out["solve_ms"] = ms
+                out["iters"] = len(holder["hist"])
+                out["sweeps"] = len(holder["hist"]) + 3
+                out["residual"] = float(holder["hist"][-1]) \
+                    if len(holder["hist"]) else 0.0
+            out["phi"] = s.phi()
+            _ = np
     finally:
         s.close()
-    return dict(ms_per_step=ms_steps / steps, ms_per_sweep=ms_sweeps / steps,
-                n_steps=info["n_steps"], min_levels=info["min_levels"],
-                psi_gb=info["psi_bytes"] / 1e9, setup_s=setup_s)
+    return out
 
 
 def cpu_leg(p, n_angles, threads):
@@ -129,29 +140,35 @@ def counter_run(p, precision, sweeps):
     hip.hipDeviceSynchronize()
     hip.hipFree(a)
     hip.hipFree(b)
-    print(f"# counter run: {sweeps} sweeps, calibration copy {nbytes} B",
-          file=sys.stderr, flush=True)
+    log(f"counter run: {sweeps} sweeps, calibration copy {nbytes} B")
 
 
-def traffic_from_profiles(workload):
+def traffic_from_profiles(workload, precision):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        return json.load(f).get(precision, {}).get("hbm_bytes_per_launch")
+
+
+def maxrel(a, b):
+    import numpy as np
+    return float(np.abs(a - b).max() / np.abs(b).max())
 
 
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="precision",
                     choices=["plumbing", "plain", "precision", "scattering"])
     ap.add_argument("--cpu-angles", type=int, default=0,
                     help="angles in the CPU-baseline sample (0: 2 x cores)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="skip the two solves to 1e-10 (steps only)")
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--counter-run", choices=["fp64", "fp32"],
                     help="only a short run for rocprofv3 --pmc")
     args = ap.parse_args()
@@ -163,62 +180,99 @@ def main() -> int:
 
     from tycho2_amd import problems
     from tycho2_amd.sweeper import FP32, FP64
+    from tycho2_amd.timing import HipEvents
     t0 = time.time()
     p = getattr(problems, args.config)()
     n = round((p.nc / 6) ** (1.0 / 3.0))
-    gen_s = time.time() - t0
-    print(f"# {args.config}: {p.nc} cells x {p.na} angles x {p.ng} groups "
-          f"= {p.solves_per_sweep} solves/sweep (mesh {gen_s:.1f}s)",
-          file=sys.stderr, flush=True)
+    log(f"{args.config}: {p.nc} cells x {p.na} angles x {p.ng} groups = "
+        f"{p.solves_per_sweep} solves/sweep (mesh {time.time() - t0:.1f}s)")
     if args.counter_run:
         counter_run(p, FP64 if args.counter_run == "fp64" else FP32,
                     max(1, args.steps))
         return 0
     ev = HipEvents()
-    r64 = gpu_leg(p, FP64, args.steps, args.warmup, ev)
-    print(f"# fp64: {r64}", file=sys.stderr, flush=True)
-    r32 = None
-    if not args.no_fp32:
-        r32 = gpu_leg(p, FP32, args.steps, args.warmup, ev)
-        print(f"# fp32: {r32}", file=sys.stderr, flush=True)
+    solve = not args.no_solve
+    tune = not args.no_tune
+    r64 = path_leg(p, FP64, args.steps, args.warmup, ev, tune, solve)
+    log("fp64: " + str({k: v for k, v in r64.items() if k != "phi"}))
+    r32 = path_leg(p, FP32, args.steps, args.warmup, ev, tune, solve)
+    log("fp32: " + str({k: v for k, v in r32.items()
+                        if k not in ("phi", "phi1")}))
 
     solves = p.solves_per_sweep
-    value = solves / (r64["ms_per_step"] * 1e-3)
-    b64 = alg_bytes_per_solve(8, n, p.na)
-    launch_ms = r64["ms_per_sweep"] / r64["n_steps"]
-    achieved = solves * b64 / (r64["ms_per_sweep"] * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": "sweep_step_kernel<double,16,1>",
-            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(args.config),
-            "alg_bytes_per_solve": round(b64, 3),
-            "alg_bytes_per_launch": round(solves * b64 / r64["n_steps"]),
-            "avg_launch_ms": round(launch_ms, 5),
-            "launches_per_sweep": r64["n_steps"]}
-    out = {"metric": "DG cell-angle-group solves/s (FP64 sweep, source-"
-                     "iteration step)",
-           "value": round(value, 1), "unit": "solves/s", "n_gpus": 1,
+    b64, b32 = alg_bytes_per_solve(8, n, p.na), alg_bytes_per_solve(4, n, p.na)
+
+    def roof(r, width, bytes_per_solve, kernel):
+        ach = solves * bytes_per_solve / (r["ms_per_sweep"] * 1e-3) / 1e9
+        return {"bound": "hbm", "kernel": kernel, "achieved": round(ach, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles(args.config,
+                                                 f"fp{8 * width}"),
+                "alg_bytes_per_solve": round(bytes_per_solve, 3),
+                "alg_bytes_per_launch": round(solves * bytes_per_solve /
+                                              r["n_steps"]),
+                "avg_launch_ms": round(r["ms_per_sweep"] / r["n_steps"], 5),
+                "launches_per_sweep": r["n_steps"]}
+
+    plain = {"solves_per_s_step": round(solves / (r64["ms_per_step"] * 1e-3)),
+             "solves_per_s_sweep": round(solves /
+                                         (r64["ms_per_sweep"] * 1e-3)),
+             "ms_per_sweep": round(r64["ms_per_sweep"], 3),
+             "tuned": r64.get("tuned"),
+             "roofline": roof(r64, 8, b64, "sweep_step_kernel<double,16,1>")}
+    refined = {"solves_per_s_step": round(solves / (r32["ms_per_step"] * 1e-3)),
+               "solves_per_s_sweep": round(solves /
+                                           (r32["ms_per_sweep"] * 1e-3)),
+               "ms_per_sweep": round(r32["ms_per_sweep"], 3),
+               "tuned": r32.get("tuned"),
+               "roofline": roof(r32, 4, b32, "sweep_step_kernel<float,16,1>")}
+    value = plain["solves_per_s_step"]
+    metric_path = "FP64 plain, source-iteration step"
+    ms_per_step = r64["ms_per_step"]
+    dtype = "f64"
+    roofline = plain["roofline"]
+    if solve:
+        plain.update(solve_s=round(r64["solve_ms"] / 1e3, 3),
+                     gmres_iters=r64["iters"], sweeps=r64["sweeps"],
+                     solves_per_s_solve=round(r64["sweeps"] * solves /
+                                              (r64["solve_ms"] * 1e-3)))
+        e1 = maxrel(r32["phi1"], r64["phi"])
+        e2 = maxrel(r32["phi"], r64["phi"])
+        refined.update(solve_s=round(r32["solve_ms"] / 1e3, 3),
+                       sweeps=r32["sweeps"],
+                       solves_per_s_solve=round(r32["sweeps"] * solves /
+                                                (r32["solve_ms"] * 1e-3)),
+                       phi_maxrel_err_after_fp32_stage=e1,
+                       phi_maxrel_err_after_correction=e2,
+                       gmres_iters=[r32["refine"]["iters_sweep"],
+                                    r32["refine"]["iters_correction"]])
+        refined["speedup_solves_per_s"] = round(
+            refined["solves_per_s_solve"] / plain["solves_per_s_solve"], 3)
+        refined["speedup_time_to_solution"] = round(
+            plain["solve_s"] / refined["solve_s"], 3)
+        if e2 <= 1e-10:
+            value = refined["solves_per_s_solve"]
+            metric_path = ("FP32 sweeps + FP64 defect correction, whole "
+                           "solve to 1e-10 (max-rel phi err vs FP64)")
+            ms_per_step = r32["solve_ms"] / r32["sweeps"]
+            dtype = "f32+f64"
+            roofline = refined["roofline"]
+    out = {"metric": "DG cell-angle-group solves/s, FP64 plain vs "
+                     "FP32+FP64-refined at <=1e-10 phi err "
+                     f"[value: {metric_path}]",
+           "value": float(value), "unit": "solves/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": round(r64["ms_per_step"], 4),
+           "ms_per_step": round(ms_per_step, 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64", "data": "synthetic",
+           "dtype": dtype, "data": "synthetic",
            "config": {"workload": f"{args.config}: {p.nc} tets (Kuhn cube "
                                   f"{n}^3 x 6, jitter 0.15h), {p.na} angles "
                                   f"(S{p.sn}), {p.ng} groups",
                       "cells": p.nc, "angles": p.na, "groups": p.ng,
-                      "solves_per_step": solves},
-           "roofline": roof,
-           "sweep_only_solves_per_s": round(
-               solves / (r64["ms_per_sweep"] * 1e-3), 1)}
-    if r32:
-        b32 = alg_bytes_per_solve(4, n, p.na)
-        a32 = solves * b32 / (r32["ms_per_sweep"] * 1e-3) / 1e9
-        out["fp32"] = {"solves_per_s_step": round(
-                           solves / (r32["ms_per_step"] * 1e-3), 1),
-                       "solves_per_s_sweep": round(
-                           solves / (r32["ms_per_sweep"] * 1e-3), 1),
-                       "achieved_gbs": round(a32, 1),
-                       "frac": round(a32 / HBM_PEAK_GBS, 4)}
+                      "solves_per_sweep": solves},
+           "roofline": roofline, "plain_fp64": plain,
+           "refined_fp32_fp64": refined}
     if not args.no_cpu:
         cores = min(16, len(os.sched_getaffinity(0)))
         na_cpu = args.cpu_angles or min(p.na, 2 * cores)

@@ -45,6 +45,7 @@ extern "C" {
 #define T2_ERR_STATE -4    /* call not valid in the handle's current state */
 #define T2_ERR_NODEV -5    /* no usable GPU                                */
 #define T2_ERR_NOCONV -6   /* iteration hit its limit without converging   */
+#define T2_ERR_TIMEOUT -7  /* a ready-queue wait ran past its poll limit   */
 
 /* sweep arithmetic (also the angular-flux storage type) */
 #define T2_FP64 0
@@ -119,6 +120,21 @@ int t2_set_launch(t2_sweeper *s, int items_per_block, int groups_per_lane);
  * do not depend on it. */
 int t2_set_schedule(t2_sweeper *s, int angle_batches);
 
+/* Sweep engine (GraphTraverser::traverse, src/GraphTraverser.cc:467-759):
+ *   T2_ENGINE_LEVELS  one kernel launch per schedule step, the whole sweep
+ *                     captured once in a hipGraph and replayed (default);
+ *   T2_ENGINE_QUEUE   one persistent launch: blocks take chunks of items in
+ *                     schedule order from a device ticket counter, each item
+ *                     waits on its atomic pending-upwind counter, solves, and
+ *                     decrements the counters of the cells that read it.
+ * Results are bitwise the same.  spin_limit bounds every queue wait (polls;
+ * 0 = default); a wait that runs out abandons the sweep, and the next
+ * synchronising call on the handle returns T2_ERR_TIMEOUT (the counters are
+ * re-armed, so later sweeps are valid). */
+#define T2_ENGINE_LEVELS 0
+#define T2_ENGINE_QUEUE 1
+int t2_set_engine(t2_sweeper *s, int engine, int64_t spin_limit);
+
 /* ------------------------------------------------------------ drop-in sweep
  * SweeperAbstract::sweep(psi, source, zeroPsiBound) on host arrays in the
  * reference layout: psi = L^-1 source for all angles and groups.  psi is
@@ -154,7 +170,7 @@ int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,
  *      psi32 = L^-1 (q + M S phi) by one more sweep, phi1 = D psi32 (FP64).
  *   2. r = q + M S phi1 - L psi32, the defect of the coupled equations,
  *      evaluated in FP64 from FP64 cell coefficients (a parallel operator
- *      apply, no sweep), stored in FP32.
+ *      apply, no sweep), stored in FP32 as a nodal source density.
  *   3. GMRES on the correction (L - M S D) delta = r with FP32 sweeps, to
  *      rtol_correction; phi = phi1 + D delta in FP64.
  * Memory: psi32 plus an FP32 residual (psi-sized each).  phi1 (host,
@@ -180,6 +196,23 @@ int t2_sweep_step(t2_sweeper *s);
 /* One sweep psi = L^-1 (current total source) without the source update or
  * the moment (times the sweep kernels alone). */
 int t2_sweep_only(t2_sweeper *s);
+
+/* ------------------------------------------------------------ batches
+ * Several independent problems (handles on one device with the same group
+ * count and precision, isotropic sources) swept together: their schedules
+ * are merged step by step, so one kernel launch per merged step covers every
+ * member (small problems no longer pay a launch per level each).  Members
+ * keep their own psi, sources and phi; results equal the members' own
+ * sweeps bit for bit.  No reference counterpart (one problem per run). */
+typedef struct t2_batch t2_batch;
+int t2_batch_create(t2_batch **out, t2_sweeper *const *members, int count);
+int t2_batch_destroy(t2_batch *b);
+/* t2_sweep_step of every member with one merged sweep */
+int t2_batch_sweep_step(t2_batch *b);
+int t2_batch_sweep_only(t2_batch *b);
+int t2_batch_info(const t2_batch *b, int64_t *n_steps, int64_t *n_items);
+int t2_batch_synchronize(t2_batch *b);
+int t2_batch_get_stream(const t2_batch *b, void **stream);
 
 /* Copy results to the host in the reference layouts. */
 int t2_get_phi(const t2_sweeper *s, double *phi);

@@ -1,0 +1,125 @@
+"""Ready-queue sweep engine (one persistent launch with atomic upwind
+counters) against the per-step level engine and the oracle.
+
+The two engines run the same cell solve on the same records, so every result
+must agree bit for bit; the wait bound of the queue is exercised from the
+host with a tiny poll limit, which makes the kernel give up and exit.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from tycho2_amd import mesh as meshmod
+from tycho2_amd.quadrature import product_quadrature
+from tycho2_amd.sweeper import (ENGINE_LEVELS, ENGINE_QUEUE, ERR_TIMEOUT,
+                                FP32, FP64, SweeperHIP, T2Error)
+
+pytestmark = pytest.mark.gpu
+
+SPIN = 1 << 20      # polls per wait: far above any wait of these sizes
+
+
+def problem(n, sn, ng, seed, jitter=0.15):
+    sm = meshmod.kuhn_cube(n, jitter, seed=seed)
+    conn = meshmod.tycho_connectivity(sm)
+    omega, wt = (orc.quadrature(sn) if sn == 2 else product_quadrature(sn))
+    rng = np.random.default_rng(seed)
+    st = rng.uniform(0.5, 3.0, (sm.n_cells, ng))
+    ss = st * rng.uniform(0.1, 0.9, (sm.n_cells, ng))
+    q = rng.uniform(0.0, 2.0, (sm.n_cells, 4, ng))
+    return sm, conn, omega, wt, st, ss, q
+
+
+def run_steps(args, precision, engine, steps=3, batches=1, launch=None):
+    sm, conn, omega, wt, st, ss, q = args
+    ng = q.shape[2]
+    with SweeperHIP(sm, conn, omega, wt, ng, st, ss, precision) as s:
+        s.set_engine(engine, SPIN)
+        if batches != 1:
+            s.set_schedule(batches)
+        if launch:
+            s.set_launch(*launch)
+        s.set_source_isotropic(q)
+        for _ in range(steps):
+            s.sweep_step()
+        return s.phi(), s.psi_angle(omega.shape[0] - 1)
+
+
+@pytest.mark.parametrize("precision,ng", [(FP64, 1), (FP64, 4), (FP64, 16),
+                                          (FP32, 1), (FP32, 16)])
+def test_queue_matches_levels_bitwise(precision, ng):
+    print(f"queue vs levels precision={precision} ng={ng}", flush=True)
+    args = problem(5, 4, ng, seed=40 + ng)
+    ref = run_steps(args, precision, ENGINE_LEVELS)
+    got = run_steps(args, precision, ENGINE_QUEUE)
+    assert np.array_equal(ref[0], got[0])
+    assert np.array_equal(ref[1], got[1])
+
+
+@pytest.mark.parametrize("batches,launch", [(2, None), (1, (8, 2)),
+                                            (4, (4, 1))])
+def test_queue_schedules_and_shapes_bitwise(batches, launch):
+    print(f"queue batches={batches} launch={launch}", flush=True)
+    args = problem(4, 4, 16, seed=52)
+    ref = run_steps(args, FP64, ENGINE_LEVELS)
+    got = run_steps(args, FP64, ENGINE_QUEUE, batches=batches, launch=launch)
+    assert np.array_equal(ref[0], got[0])
+    assert np.array_equal(ref[1], got[1])
+
+
+def test_queue_lagged_sweep_matches_oracle():
+    """Cyclic mesh: lagged faces read the pre-sweep psi in the queue too."""
+    print("queue cyclic sweep", flush=True)
+    sm = meshmod.kuhn_cube(3, 0.2, seed=3)
+    conn = meshmod.tycho_connectivity(sm)
+    omega, wt = orc.quadrature(2)
+    rng = np.random.default_rng(8)
+    st = rng.uniform(1.0, 2.0, (sm.n_cells, 1))
+    src = rng.uniform(0.1, 1.0, (sm.n_cells, omega.shape[0], 4, 1))
+    prev = rng.uniform(0.0, 1.0, src.shape)
+    psi = prev.copy()
+    with SweeperHIP(sm, conn, omega, wt, 1, st, 0.5 * st, FP64) as s:
+        s.set_engine(ENGINE_QUEUE, SPIN)
+        assert s.info()["n_lagged"] > 0
+        s.sweep(psi, src)
+    ref, _ = orc.sweep(orc.build_mesh(sm), omega, 1, st, src, psi=prev.copy())
+    assert np.abs(psi - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_queue_krylov_bitwise():
+    print("queue krylov", flush=True)
+    sm, conn, omega, wt, st, ss, q = problem(4, 4, 4, seed=61)
+    out = []
+    for engine in (ENGINE_LEVELS, ENGINE_QUEUE):
+        with SweeperHIP(sm, conn, omega, wt, 4, st, ss, FP64) as s:
+            s.set_engine(engine, SPIN)
+            s.set_source_isotropic(q)
+            hist = s.solve_krylov(200, 1e-10, 20)
+            out.append((s.phi(), len(hist)))
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
+
+
+def test_queue_wait_limit_abandons_and_recovers():
+    """With a poll limit of 1 some wait runs out: the kernel exits, the next
+    synchronising call reports it, and later sweeps are exact again."""
+    print("queue wait limit", flush=True)
+    args = problem(6, 4, 4, seed=70)
+    ref = run_steps(args, FP64, ENGINE_LEVELS, steps=1)
+    sm, conn, omega, wt, st, ss, q = args
+    with SweeperHIP(sm, conn, omega, wt, 4, st, ss, FP64) as s:
+        s.set_source_isotropic(q)
+        s.set_engine(ENGINE_QUEUE, 1)
+        timeouts = 0
+        for _ in range(5):
+            try:
+                s.sweep_only()
+                s.synchronize()
+            except T2Error as e:
+                assert e.code == ERR_TIMEOUT
+                timeouts += 1
+        assert timeouts >= 1
+        s.set_engine(ENGINE_QUEUE, SPIN)
+        s.set_phi(np.zeros_like(ref[0]))
+        s.sweep_step()
+        assert np.array_equal(s.phi(), ref[0])

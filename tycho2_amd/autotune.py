@@ -1,0 +1,67 @@
+"""Autotuner of the sweep schedule and launch shape.
+
+For one sweeper handle it times full sweeps (HIP events on the engine's
+stream) over
+  * angle batches  -- how many octant-ordered groups of angles are swept one
+    after the other (smaller per-step working set, more steps),
+  * cells per block and groups per lane -- the launch shape of the sweep
+    kernels,
+keeps the fastest setting on the handle and returns the measurements.
+Results do not depend on any of these knobs (bitwise, tested).
+"""
+from __future__ import annotations
+
+from .timing import HipEvents, time_region
+
+BATCHES = (1, 2, 4, 8, 16)
+BLOCKS = (256, 128, 64)          # threads per block
+GROUPS_PER_LANE = (1, 2)
+
+
+def _time_sweep(ev, s, reps):
+    s.sweep_only()
+    s.synchronize()
+    return time_region(ev, s, s.sweep_only, reps) / reps
+
+
+def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
+         groups_per_lane=GROUPS_PER_LANE, verbose: bool = False):
+    """Tune handle `s` in place; returns (best, trials) where best is a dict
+    {batches, block, groups_per_lane, ms} and trials lists every timing."""
+    ev = HipEvents()
+    ng = s.ng
+    trials = []
+    best = None
+    # schedule first (it rebuilds the records), with the default launch
+    for nb in batches:
+        if nb > s.na:
+            continue
+        s.set_schedule(nb)
+        s.set_launch(256 // ng if ng <= 256 else 1, 1)
+        ms = _time_sweep(ev, s, reps)
+        t = dict(batches=nb, block=256, groups_per_lane=1, ms=ms)
+        trials.append(t)
+        if verbose:
+            print(t, flush=True)
+        if best is None or ms < best["ms"]:
+            best = t
+    s.set_schedule(best["batches"])
+    for gpl in groups_per_lane:
+        if gpl > ng:
+            continue
+        lanes = ng // gpl
+        for block in blocks:
+            if block % lanes or (block, gpl) == (256, 1):
+                continue
+            s.set_launch(block // lanes, gpl)
+            ms = _time_sweep(ev, s, reps)
+            t = dict(batches=best["batches"], block=block,
+                     groups_per_lane=gpl, ms=ms)
+            trials.append(t)
+            if verbose:
+                print(t, flush=True)
+            if ms < best["ms"]:
+                best = t
+    lanes = ng // best["groups_per_lane"]
+    s.set_launch(best["block"] // lanes, best["groups_per_lane"])
+    return best, trials

@@ -82,6 +82,17 @@ struct t2_sweeper {
 
     std::vector<GraphEntry> graphs;
 
+    // ready-queue engine, built on first use for the schedule and launch
+    // shape (n_chunks < 0: not built)
+    int engine = T2_ENGINE_LEVELS;
+    int64_t spin_limit = 0;
+    int64_t n_chunks = -1, queue_grid = 0;
+    int64_t *d_chunk_off = nullptr;
+    int32_t *d_pending = nullptr;    // [na][nc]
+    int4 *d_down = nullptr;          // [n_items]
+    unsigned long long *d_ticket = nullptr;
+    int32_t *d_abort = nullptr;
+
     int64_t n_phi() const { return nc * 4 * ng; }
     int64_t n_psi() const { return na * n_phi(); }
     size_t tsize() const { return precision == T2_FP64 ? 8 : 4; }
@@ -157,6 +168,21 @@ void drop_graphs(t2_sweeper *s)
     s->graphs.clear();
 }
 
+// Default poll limit of a ready-queue wait (polls of ~1 us: seconds, far
+// beyond any wait of a correct sweep, which is bounded by the sweep itself).
+constexpr int64_t kDefaultSpinLimit = int64_t(1) << 22;
+
+void free_queue(t2_sweeper *s)
+{
+    const size_t n_items = size_t(s->nc) * s->na;
+    drop_graphs(s);
+    if (s->n_chunks >= 0)
+        dfree(s, s->d_chunk_off, sizeof(int64_t) * size_t(s->n_chunks + 1));
+    dfree(s, s->d_pending, sizeof(int32_t) * n_items);
+    dfree(s, s->d_down, sizeof(int4) * n_items);
+    s->n_chunks = -1;
+}
+
 // ------------------------------------------------------------------ sweeps
 template <typename T>
 int enqueue_sweep(t2_sweeper *s, const double *ang, const void *ang_t)
@@ -177,6 +203,25 @@ int enqueue_sweep(t2_sweeper *s, const double *ang, const void *ang_t)
     a.psi = static_cast<T *>(s->d_psi);
     a.lag_buf = static_cast<const T *>(s->d_lag_buf);
     const auto *geo = static_cast<const t2::ItemGeo<T> *>(s->d_item_geo);
+    if (s->engine == T2_ENGINE_QUEUE) {
+        a.geo = geo;
+        a.topo = s->d_item_topo;
+        a.n_items = s->nc * s->na;
+        t2::QueueState q{};
+        q.chunk_off = s->d_chunk_off;
+        q.n_chunks = s->n_chunks;
+        q.pending = s->d_pending;
+        q.down = s->d_down;
+        q.ticket = s->d_ticket;
+        q.abort = s->d_abort;
+        q.spin_limit = s->spin_limit > 0 ? s->spin_limit : kDefaultSpinLimit;
+        // the ticket is cleared by a kernel (graph-safe), 8 bytes of zeros
+        T2_HIP(t2::launch_fill(reinterpret_cast<double *>(s->d_ticket), 0.0,
+                               1, s->stream));
+        T2_HIP(t2::launch_queue_sweep<T>(a, q, int(s->ng), s->gpl, s->block,
+                                         s->queue_grid, s->stream));
+        return T2_OK;
+    }
     for (int64_t st = 0; st < s->sched.n_steps; st++) {
         const int64_t off = s->sched.step_off[st];
         a.geo = geo + off;
@@ -194,10 +239,107 @@ int enqueue_sweep_any(t2_sweeper *s, const double *ang, const void *ang_t)
                                    : enqueue_sweep<float>(s, ang, ang_t);
 }
 
+// Ready-queue tables for the current schedule and launch shape: chunks of
+// one block's items that never span two steps, the readers of every item
+// and the pending-upwind counters.
+int build_queue(t2_sweeper *s)
+{
+    free_queue(s);
+    const int64_t nc = s->nc, na = s->na;
+    const int64_t n_items = nc * na;
+    if (n_items >= (int64_t(1) << 31))
+        return fail(T2_ERR_ARG, "ready queue: more than 2^31 items");
+    const int64_t per_chunk = s->block / (s->ng / s->gpl);
+    std::vector<int64_t> off(1, 0);
+    for (int64_t st = 0; st < s->sched.n_steps; st++)
+        for (int64_t i = s->sched.step_off[st]; i < s->sched.step_off[st + 1];
+             i += per_chunk)
+            off.push_back(std::min(i + per_chunk, s->sched.step_off[st + 1]));
+    const int64_t n_chunks = int64_t(off.size()) - 1;
+    T2_TRY(dalloc_t(s, &s->d_chunk_off, sizeof(int64_t) * off.size()));
+    s->n_chunks = n_chunks;  // from here on free_queue releases the table
+    T2_TRY(upload(s->d_chunk_off, off.data(), off.size()));
+    T2_TRY(dalloc_t(s, &s->d_pending, sizeof(int32_t) * size_t(n_items)));
+    T2_TRY(dalloc_t(s, &s->d_down, sizeof(int4) * size_t(n_items)));
+    if (!s->d_ticket) {
+        T2_TRY(dalloc_t(s, &s->d_ticket, sizeof(unsigned long long)));
+        T2_TRY(dalloc_t(s, &s->d_abort, sizeof(int32_t)));
+        T2_HIP(hipMemset(s->d_abort, 0, sizeof(int32_t)));
+    }
+    int32_t *pos = nullptr, *dcount = nullptr;
+    int *d_err = nullptr;
+    auto cleanup = [&]() {
+        dfree(s, pos, sizeof(int32_t) * size_t(n_items));
+        dfree(s, dcount, sizeof(int32_t) * size_t(n_items));
+        dfree(s, d_err, sizeof(int));
+    };
+    int rc = T2_OK, err = 0;
+    if ((rc = dalloc_t(s, &pos, sizeof(int32_t) * size_t(n_items))) ||
+        (rc = dalloc_t(s, &dcount, sizeof(int32_t) * size_t(n_items))) ||
+        (rc = dalloc_t(s, &d_err, sizeof(int)))) {
+        cleanup();
+        return rc;
+    }
+    hipError_t e = hipMemsetAsync(d_err, 0, sizeof(int), s->stream);
+    if (e == hipSuccess)
+        e = t2::launch_queue_links(s->d_item_topo, n_items, nc, pos, dcount,
+                                   s->d_down, s->d_pending, d_err, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost,
+                           s->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s->stream);
+    cleanup();
+    if (e != hipSuccess)
+        return fail(T2_ERR_HIP, std::string("ready-queue set-up: ") +
+                                    hipGetErrorString(e));
+    if (err)
+        return fail(T2_ERR_ARG, "ready-queue set-up: a cell feeds more than "
+                                "four cells of one angle");
+    int64_t grid = 0;
+    if (s->precision == T2_FP64)
+        T2_HIP(t2::queue_grid<double>(int(s->ng), s->gpl, s->block, &grid));
+    else
+        T2_HIP(t2::queue_grid<float>(int(s->ng), s->gpl, s->block, &grid));
+    s->queue_grid = grid;
+    return T2_OK;
+}
+
+// After an abandoned ready-queue sweep: reports it once and re-arms the
+// counters so that later sweeps are valid.
+int check_abort(t2_sweeper *s)
+{
+    if (s->engine != T2_ENGINE_QUEUE || !s->d_abort)
+        return T2_OK;
+    int32_t flag = 0;
+    T2_HIP(hipStreamSynchronize(s->stream));
+    T2_HIP(hipMemcpy(&flag, s->d_abort, sizeof(flag), hipMemcpyDeviceToHost));
+    if (!flag)
+        return T2_OK;
+    if (s->d_pending)
+        T2_HIP(t2::launch_queue_rearm(s->d_item_topo, s->nc * s->na, s->nc,
+                                      s->d_pending, s->stream));
+    T2_HIP(hipMemsetAsync(s->d_abort, 0, sizeof(int32_t), s->stream));
+    T2_HIP(hipStreamSynchronize(s->stream));
+    return fail(T2_ERR_TIMEOUT,
+                "a ready-queue wait ran past its poll limit; the sweep was "
+                "abandoned and its results are invalid");
+}
+
+int after_queue(t2_sweeper *s, int rc)
+{
+    if (!s)
+        return rc;
+    const int a = check_abort(s);
+    return a != T2_OK ? a : rc;
+}
+
 // psi = L^-1 (src + ang + ang_t): every step of the schedule, captured once
 // per source combination into a hipGraph and replayed.
 int run_sweep(t2_sweeper *s, const double *ang, const void *ang_t = nullptr)
 {
+    if (s->engine == T2_ENGINE_QUEUE && s->n_chunks < 0)
+        T2_TRY(build_queue(s));
     if (!s->use_graph)
         return enqueue_sweep_any(s, ang, ang_t);
     hipGraphExec_t exec = nullptr;
@@ -327,6 +469,7 @@ void free_schedule(t2_sweeper *s)
     const size_t n_items = size_t(s->nc) * s->na;
     const int64_t n_lag = int64_t(s->sched.lagged.size());
     drop_graphs(s);
+    free_queue(s);
     dfree(s, s->d_item_geo, geo_record_size(s) * n_items);
     dfree(s, s->d_item_geo64, sizeof(t2::ItemGeo<double>) * n_items);
     dfree(s, s->d_item_topo, sizeof(t2::ItemTopo) * n_items);
@@ -727,8 +870,11 @@ int t2_set_launch(t2_sweeper *s, int items_per_block, int groups_per_lane)
                         "multiple of 64 in [64, 256]");
         block = int(lanes);
     }
-    if (block != s->block || gpl != s->gpl)
-        drop_graphs(s);
+    if (block != s->block || gpl != s->gpl) {
+        T2_TRY(set_device(s));
+        T2_HIP(hipStreamSynchronize(s->stream));
+        free_queue(s);
+    }
     s->block = block;
     s->gpl = gpl;
     return T2_OK;
@@ -747,7 +893,7 @@ int t2_set_schedule(t2_sweeper *s, int angle_batches)
     return build_schedule_device(s);
 }
 
-int t2_sweep_host(t2_sweeper *s, double *psi, const double *source,
+static int sweep_host_body(t2_sweeper *s, double *psi, const double *source,
                   int zero_psi_bound)
 {
     (void)zero_psi_bound;  // no partition faces on one device
@@ -845,7 +991,7 @@ int t2_sweep_only(t2_sweeper *s)
     return run_sweep(s, s->d_qang);
 }
 
-int t2_solve_source_iteration(t2_sweeper *s, int64_t iter_max, double err_max,
+static int solve_si_body(t2_sweeper *s, int64_t iter_max, double err_max,
                               int64_t *n_iter, double *err_hist)
 {
     t2::clear_error();
@@ -872,7 +1018,7 @@ int t2_solve_source_iteration(t2_sweeper *s, int64_t iter_max, double err_max,
     return T2_OK;
 }
 
-int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,
+static int solve_krylov_body(t2_sweeper *s, int64_t iter_max, double rtol,
                     int64_t restart, int64_t *n_iter, double *res_hist)
 {
     t2::clear_error();
@@ -896,7 +1042,7 @@ int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,
     return T2_OK;
 }
 
-int t2_solve_refined(t2_sweeper *s, int64_t iter_max, double rtol_sweep,
+static int solve_refined_body(t2_sweeper *s, int64_t iter_max, double rtol_sweep,
                      double rtol_correction, int64_t restart,
                      t2_refine_info *info, double *phi1)
 {
@@ -980,7 +1126,7 @@ int t2_get_phi(const t2_sweeper *s, double *phi)
     T2_HIP(hipStreamSynchronize(s->stream));
     T2_HIP(hipMemcpy(phi, s->d_phi, sizeof(double) * s->n_phi(),
                      hipMemcpyDeviceToHost));
-    return T2_OK;
+    return check_abort(const_cast<t2_sweeper *>(s));
 }
 
 int t2_set_phi(t2_sweeper *s, const double *phi)
@@ -1043,7 +1189,53 @@ int t2_synchronize(t2_sweeper *s)
     t2::clear_error();
     T2_TRY(check(s));
     T2_HIP(hipStreamSynchronize(s->stream));
+    return check_abort(s);
+}
+
+int t2_set_engine(t2_sweeper *s, int engine, int64_t spin_limit)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (engine != T2_ENGINE_LEVELS && engine != T2_ENGINE_QUEUE)
+        return fail(T2_ERR_ARG, "engine must be T2_ENGINE_LEVELS or "
+                                "T2_ENGINE_QUEUE");
+    if (spin_limit < 0)
+        return fail(T2_ERR_ARG, "spin_limit < 0");
+    T2_HIP(hipStreamSynchronize(s->stream));
+    if (engine != s->engine || spin_limit != s->spin_limit)
+        drop_graphs(s);
+    s->engine = engine;
+    s->spin_limit = spin_limit;
     return T2_OK;
+}
+
+int t2_sweep_host(t2_sweeper *s, double *psi, const double *source,
+                  int zero_psi_bound)
+{
+    return after_queue(s, sweep_host_body(s, psi, source, zero_psi_bound));
+}
+
+int t2_solve_source_iteration(t2_sweeper *s, int64_t iter_max, double err_max,
+                              int64_t *n_iter, double *err_hist)
+{
+    return after_queue(s, solve_si_body(s, iter_max, err_max, n_iter,
+                                        err_hist));
+}
+
+int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,
+                    int64_t restart, int64_t *n_iter, double *res_hist)
+{
+    return after_queue(s, solve_krylov_body(s, iter_max, rtol, restart,
+                                            n_iter, res_hist));
+}
+
+int t2_solve_refined(t2_sweeper *s, int64_t iter_max, double rtol_sweep,
+                     double rtol_correction, int64_t restart,
+                     t2_refine_info *info, double *phi1)
+{
+    return after_queue(s, solve_refined_body(s, iter_max, rtol_sweep,
+                                             rtol_correction, restart, info,
+                                             phi1));
 }
 
 int t2_get_stream(const t2_sweeper *s, void **stream)
@@ -1052,6 +1244,340 @@ int t2_get_stream(const t2_sweeper *s, void **stream)
     if (!s || !stream)
         return fail(T2_ERR_ARG, "t2_get_stream: bad argument");
     *stream = static_cast<void *>(s->stream);
+    return T2_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================== batch
+struct t2_batch {
+    std::vector<t2_sweeper *> members;
+    int device = 0;
+    int precision = T2_FP64;
+    int64_t ng = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t order = nullptr;          // orders member and batch streams
+    std::vector<int64_t> step_off;       // merged schedule
+    int64_t n_items = 0;
+    void *d_geo = nullptr;
+    t2::ItemTopo *d_topo = nullptr;
+    void *d_desc = nullptr;
+    size_t geo_bytes = 0, desc_bytes = 0;
+    hipGraphExec_t graph = nullptr;
+    std::vector<std::vector<const void *>> snap;  // members' buffers at create
+};
+
+namespace {
+
+std::vector<const void *> member_buffers(const t2_sweeper *m)
+{
+    return {m->d_item_geo, m->d_item_topo, m->d_psi, m->d_src, m->d_lag_buf,
+            m->d_lag_key, m->d_sigt, m->d_qang,
+            reinterpret_cast<const void *>(intptr_t(m->sched.n_steps)),
+            reinterpret_cast<const void *>(intptr_t(m->block)),
+            reinterpret_cast<const void *>(intptr_t(m->gpl))};
+}
+
+// A member whose schedule, launch shape, buffers or source kind changed since
+// the batch was made would be swept with stale records.
+int batch_check(const t2_batch *b)
+{
+    if (!b)
+        return fail(T2_ERR_ARG, "null batch");
+    for (size_t i = 0; i < b->members.size(); i++)
+        if (member_buffers(b->members[i]) != b->snap[i])
+            return fail(T2_ERR_STATE,
+                        "batch member " + std::to_string(i) +
+                            " changed its schedule, launch shape or source "
+                            "since t2_batch_create");
+    return T2_OK;
+}
+
+int batch_fail_cleanup(t2_batch *b, int rc)
+{
+    std::string msg = t2_last_error();
+    t2_batch_destroy(b);
+    return fail(rc, msg);
+}
+
+template <typename T>
+int batch_build(t2_batch *b)
+{
+    const int nm = int(b->members.size());
+    int64_t n_steps = 0;
+    for (auto *m : b->members)
+        n_steps = std::max(n_steps, m->sched.n_steps);
+    std::vector<int64_t> count(n_steps, 0);
+    for (auto *m : b->members)
+        for (int64_t st = 0; st < m->sched.n_steps; st++)
+            count[st] += m->sched.step_off[st + 1] - m->sched.step_off[st];
+    b->step_off.assign(n_steps + 1, 0);
+    for (int64_t st = 0; st < n_steps; st++)
+        b->step_off[st + 1] = b->step_off[st] + count[st];
+    b->n_items = b->step_off[n_steps];
+    b->geo_bytes = sizeof(t2::ItemGeo<T>) * size_t(b->n_items);
+    T2_HIP(hipMalloc(&b->d_geo, b->geo_bytes ? b->geo_bytes : 16));
+    T2_HIP(hipMalloc(reinterpret_cast<void **>(&b->d_topo),
+                     sizeof(t2::ItemTopo) * size_t(std::max<int64_t>(1, b->n_items))));
+    std::vector<int64_t> filled(n_steps, 0);
+    std::vector<t2::ProblemDesc<T>> desc(nm);
+    int64_t *d_off = nullptr;
+    T2_HIP(hipMalloc(reinterpret_cast<void **>(&d_off),
+                     sizeof(int64_t) * 2 * size_t(n_steps + 1)));
+    for (int i = 0; i < nm; i++) {
+        t2_sweeper *m = b->members[i];
+        const int64_t ms = m->sched.n_steps;
+        std::vector<int64_t> dst(ms + 1, 0);
+        for (int64_t st = 0; st < ms; st++) {
+            dst[st] = b->step_off[st] + filled[st];
+            filled[st] += m->sched.step_off[st + 1] - m->sched.step_off[st];
+        }
+        hipError_t e = hipMemcpy(d_off, m->sched.step_off.data(),
+                                 sizeof(int64_t) * (ms + 1),
+                                 hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(d_off + ms + 1, dst.data(),
+                          sizeof(int64_t) * (ms + 1), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = t2::launch_merge_items<T>(
+                static_cast<const t2::ItemGeo<T> *>(m->d_item_geo),
+                m->d_item_topo, d_off, d_off + ms + 1, ms,
+                m->sched.step_off[ms], int32_t(i),
+                static_cast<t2::ItemGeo<T> *>(b->d_geo), b->d_topo, b->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(d_off);
+            return fail(T2_ERR_HIP, std::string("batch merge: ") +
+                                        hipGetErrorString(e));
+        }
+        desc[i].nc = m->nc;
+        desc[i].sigma_mode = m->sigma_mode;
+        desc[i].sigma = static_cast<const T *>(m->d_sigt);
+        desc[i].src_iso = static_cast<const T *>(m->d_src);
+        desc[i].psi = static_cast<T *>(m->d_psi);
+        desc[i].lag_buf = static_cast<const T *>(m->d_lag_buf);
+    }
+    (void)hipFree(d_off);
+    b->desc_bytes = sizeof(t2::ProblemDesc<T>) * nm;
+    T2_HIP(hipMalloc(&b->d_desc, b->desc_bytes));
+    T2_HIP(hipMemcpy(b->d_desc, desc.data(), b->desc_bytes,
+                     hipMemcpyHostToDevice));
+    return T2_OK;
+}
+
+template <typename T>
+int batch_enqueue(t2_batch *b)
+{
+    for (auto *m : b->members) {
+        const int64_t n_lag = int64_t(m->sched.lagged.size());
+        if (n_lag > 0)
+            T2_HIP(t2::launch_lag_gather<T>(
+                m->d_lag_key, m->d_lag_up, n_lag, m->nc, int(m->ng),
+                m->d_nbr, static_cast<const T *>(m->d_psi),
+                static_cast<T *>(m->d_lag_buf), b->stream));
+    }
+    t2::SweepArgs<T> a{};
+    a.problems = static_cast<const t2::ProblemDesc<T> *>(b->d_desc);
+    const auto *geo = static_cast<const t2::ItemGeo<T> *>(b->d_geo);
+    const t2_sweeper *m0 = b->members[0];
+    for (size_t st = 0; st + 1 < b->step_off.size(); st++) {
+        a.geo = geo + b->step_off[st];
+        a.topo = b->d_topo + b->step_off[st];
+        a.n_items = b->step_off[st + 1] - b->step_off[st];
+        T2_HIP(t2::launch_sweep_step<T>(a, int(b->ng), m0->gpl, m0->block,
+                                        b->stream));
+    }
+    return T2_OK;
+}
+
+// Work already queued on the members' streams precedes the batch's work...
+int batch_after_members(t2_batch *b)
+{
+    for (auto *m : b->members) {
+        T2_HIP(hipEventRecord(b->order, m->stream));
+        T2_HIP(hipStreamWaitEvent(b->stream, b->order, 0));
+    }
+    return T2_OK;
+}
+
+// ...and later work on the members' streams follows it.
+int members_after_batch(t2_batch *b)
+{
+    T2_HIP(hipEventRecord(b->order, b->stream));
+    for (auto *m : b->members)
+        T2_HIP(hipStreamWaitEvent(m->stream, b->order, 0));
+    return T2_OK;
+}
+
+int batch_sweep(t2_batch *b)
+{
+    if (!b->graph) {
+        hipGraph_t g = nullptr;
+        T2_HIP(hipStreamBeginCapture(b->stream,
+                                     hipStreamCaptureModeThreadLocal));
+        int rc = b->precision == T2_FP64 ? batch_enqueue<double>(b)
+                                         : batch_enqueue<float>(b);
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc != T2_OK) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return rc;
+        }
+        T2_HIP(e);
+        e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        T2_HIP(e);
+    }
+    T2_HIP(hipGraphLaunch(b->graph, b->stream));
+    return T2_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int t2_batch_create(t2_batch **out, t2_sweeper *const *members, int count)
+{
+    t2::clear_error();
+    if (!out || !members || count < 1)
+        return fail(T2_ERR_ARG, "t2_batch_create: bad argument");
+    *out = nullptr;
+    for (int i = 0; i < count; i++) {
+        const t2_sweeper *m = members[i];
+        if (!m)
+            return fail(T2_ERR_ARG, "t2_batch_create: null member");
+        if (m->device != members[0]->device || m->ng != members[0]->ng ||
+            m->precision != members[0]->precision)
+            return fail(T2_ERR_ARG, "batch members need the same device, "
+                                    "group count and precision");
+        if (m->d_qang)
+            return fail(T2_ERR_ARG, "batch members need isotropic sources");
+        for (int j = 0; j < i; j++)
+            if (members[j] == m)
+                return fail(T2_ERR_ARG, "a handle appears twice in a batch");
+    }
+    auto *b = new (std::nothrow) t2_batch();
+    if (!b)
+        return fail(T2_ERR_NOMEM, "host allocation");
+    b->members.assign(members, members + count);
+    b->device = members[0]->device;
+    b->precision = members[0]->precision;
+    b->ng = members[0]->ng;
+    if (hipSetDevice(b->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) !=
+            hipSuccess) {
+        delete b;
+        return fail(T2_ERR_HIP, "stream creation failed");
+    }
+    if (hipEventCreateWithFlags(&b->order, hipEventDisableTiming) !=
+        hipSuccess)
+        return batch_fail_cleanup(b, fail(T2_ERR_HIP, "event creation"));
+    for (auto *m : b->members)
+        if (hipStreamSynchronize(m->stream) != hipSuccess)
+            return batch_fail_cleanup(b, fail(T2_ERR_HIP, "member sync"));
+    int rc = b->precision == T2_FP64 ? batch_build<double>(b)
+                                     : batch_build<float>(b);
+    if (rc != T2_OK)
+        return batch_fail_cleanup(b, rc);
+    for (auto *m : b->members)
+        b->snap.push_back(member_buffers(m));
+    *out = b;
+    return T2_OK;
+}
+
+int t2_batch_destroy(t2_batch *b)
+{
+    if (!b)
+        return T2_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream)
+        (void)hipStreamSynchronize(b->stream);
+    if (b->graph)
+        (void)hipGraphExecDestroy(b->graph);
+    if (b->d_geo)
+        (void)hipFree(b->d_geo);
+    if (b->d_topo)
+        (void)hipFree(b->d_topo);
+    if (b->d_desc)
+        (void)hipFree(b->d_desc);
+    if (b->order)
+        (void)hipEventDestroy(b->order);
+    if (b->stream)
+        (void)hipStreamDestroy(b->stream);
+    delete b;
+    return T2_OK;
+}
+
+int t2_batch_sweep_only(t2_batch *b)
+{
+    t2::clear_error();
+    T2_TRY(batch_check(b));
+    T2_HIP(hipSetDevice(b->device));
+    T2_TRY(batch_after_members(b));
+    T2_TRY(batch_sweep(b));
+    return members_after_batch(b);
+}
+
+int t2_batch_sweep_step(t2_batch *b)
+{
+    t2::clear_error();
+    T2_TRY(batch_check(b));
+    T2_HIP(hipSetDevice(b->device));
+    T2_TRY(batch_after_members(b));
+    for (auto *m : b->members) {
+        if (b->precision == T2_FP64)
+            T2_HIP(t2::launch_total_source<double>(
+                m->d_qiso, m->d_sigs, m->d_phi, m->nc, int(m->ng),
+                static_cast<double *>(m->d_src), b->stream));
+        else
+            T2_HIP(t2::launch_total_source<float>(
+                m->d_qiso, m->d_sigs, m->d_phi, m->nc, int(m->ng),
+                static_cast<float *>(m->d_src), b->stream));
+    }
+    T2_TRY(batch_sweep(b));
+    for (auto *m : b->members) {
+        std::swap(m->d_phi, m->d_phi_old);
+        if (b->precision == T2_FP64)
+            T2_HIP(t2::launch_psi_to_phi<double>(
+                static_cast<const double *>(m->d_psi), m->d_wt, m->na,
+                m->n_phi(), m->d_phi, b->stream));
+        else
+            T2_HIP(t2::launch_psi_to_phi<float>(
+                static_cast<const float *>(m->d_psi), m->d_wt, m->na,
+                m->n_phi(), m->d_phi, b->stream));
+    }
+    return members_after_batch(b);
+}
+
+int t2_batch_info(const t2_batch *b, int64_t *n_steps, int64_t *n_items)
+{
+    t2::clear_error();
+    if (!b)
+        return fail(T2_ERR_ARG, "null batch");
+    if (n_steps)
+        *n_steps = int64_t(b->step_off.size()) - 1;
+    if (n_items)
+        *n_items = b->n_items;
+    return T2_OK;
+}
+
+int t2_batch_synchronize(t2_batch *b)
+{
+    t2::clear_error();
+    if (!b)
+        return fail(T2_ERR_ARG, "null batch");
+    T2_HIP(hipSetDevice(b->device));
+    T2_HIP(hipStreamSynchronize(b->stream));
+    return T2_OK;
+}
+
+int t2_batch_get_stream(const t2_batch *b, void **stream)
+{
+    t2::clear_error();
+    if (!b || !stream)
+        return fail(T2_ERR_ARG, "t2_batch_get_stream: bad argument");
+    *stream = static_cast<void *>(b->stream);
     return T2_OK;
 }
 

@@ -9,6 +9,8 @@
 // group-contiguous memory.
 #include "t2_kernels.hpp"
 
+#include <algorithm>
+
 namespace t2 {
 namespace {
 
@@ -117,7 +119,7 @@ __global__ void build_items_kernel(const uint32_t *__restrict__ work,
     rt.cell = int32_t(c);
     rt.angle = int32_t(a);
     rt.meta = 0u;
-    rt.pad = 0;
+    rt.problem = 0;
     int n_in = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -150,26 +152,36 @@ __global__ void build_items_kernel(const uint32_t *__restrict__ work,
 }
 
 // ---------------------------------------------------------------- sweep
+// Cell solve of work item `item` (topology record t) for the groups of lane
+// gl: gl, gl + LPI, ... (LPI = G / GPL lanes per item).
 template <typename T, int G, int GPL>
-__global__ void __launch_bounds__(kMaxBlock)
-sweep_step_kernel(const SweepArgs<T> p)
+__device__ __forceinline__ void solve_item(const SweepArgs<T> &p,
+                                           int64_t item, int gl,
+                                           const ItemTopo &t)
 {
     constexpr int LPI = G / GPL;          // lanes per work item
     constexpr uint32_t CG = 4 * G;        // values per (cell, angle)
-    const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t item = tid / LPI;
-    const int gl = int(tid % LPI);
-    if (item >= p.n_items)
-        return;
-    const ItemTopo t = p.topo[item];
     // record fields as values (a local copy of the whole record would be
     // placed in scratch memory by the sigma selection below)
     const ItemGeo<T> *gp = p.geo + item;
     const T ge_a12[4] = {gp->a12[0], gp->a12[1], gp->a12[2], gp->a12[3]};
     const T ge_v20 = gp->v20;
     const T ge_sig = gp->sig;
-    const size_t plane = size_t(p.nc) * CG;
-    const T *psi_a = p.psi + size_t(t.angle) * plane;
+    int64_t nc = p.nc;
+    int sigma_mode = p.sigma_mode;
+    const T *sigma = p.sigma, *src_iso = p.src_iso, *lag_buf = p.lag_buf;
+    T *psi = p.psi;
+    if (p.problems) {
+        const ProblemDesc<T> &d = p.problems[t.problem];
+        nc = d.nc;
+        sigma_mode = d.sigma_mode;
+        sigma = d.sigma;
+        src_iso = d.src_iso;
+        lag_buf = d.lag_buf;
+        psi = d.psi;
+    }
+    const size_t plane = size_t(nc) * CG;
+    const T *psi_a = psi + size_t(t.angle) * plane;
     const uint32_t own = uint32_t(t.cell) * CG;
 
     // outgoing-face sums of the streaming matrix
@@ -185,17 +197,16 @@ sweep_step_kernel(const SweepArgs<T> p)
     for (int jg = 0; jg < GPL; jg++) {
         const int g = gl + jg * LPI;
         T sig = ge_sig;
-        if (p.sigma_mode != kSigmaCell)
-            sig = p.sigma[p.sigma_mode == kSigmaGroup
-                              ? uint32_t(g)
-                              : uint32_t(t.cell) * G + g];
+        if (sigma_mode != kSigmaCell)
+            sig = sigma[sigma_mode == kSigmaGroup ? uint32_t(g)
+                                                  : uint32_t(t.cell) * G + g];
         const T s = sig * ge_v20;
 
         // source: V/20 (q_i + sum_j q_j)  (Transport.cc:58-72)
         T q[4];
 #pragma unroll
         for (int v = 0; v < 4; v++)
-            q[v] = p.src_iso[own + v * G + g];
+            q[v] = src_iso[own + v * G + g];
         if (p.src_ang) {
             const double *qa = p.src_ang + size_t(t.angle) * plane + own + g;
 #pragma unroll
@@ -236,7 +247,7 @@ sweep_step_kernel(const SweepArgs<T> p)
                     slot++;
                 }
             } else {
-                const T *pl = p.lag_buf + size_t(-2 - u) * CG + g;
+                const T *pl = lag_buf + size_t(-2 - u) * CG + g;
 #pragma unroll
                 for (int j = 0; j < 4; j++)
                     uv[j] = (j == k) ? T(0) : pl[j * G];
@@ -259,10 +270,154 @@ sweep_step_kernel(const SweepArgs<T> p)
                                    : ge_a12[i] + s + ((P - o[i]) - o[j]);
         solve_nopivot<T>(m, b);
 
-        T *dst = p.psi + size_t(t.angle) * plane + own + g;
+        T *dst = psi + size_t(t.angle) * plane + own + g;
 #pragma unroll
         for (int v = 0; v < 4; v++)
             dst[v * G] = b[v];
+    }
+}
+
+template <typename T, int G, int GPL>
+__global__ void __launch_bounds__(kMaxBlock)
+sweep_step_kernel(const SweepArgs<T> p)
+{
+    constexpr int LPI = G / GPL;
+    const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t item = tid / LPI;
+    const int gl = int(tid % LPI);
+    if (item >= p.n_items)
+        return;
+    const ItemTopo t = p.topo[item];
+    solve_item<T, G, GPL>(p, item, gl, t);
+}
+
+// ---------------------------------------------------------------- ready queue
+// The whole sweep in one persistent launch.  Blocks take chunks of work items
+// from a global ticket counter in step order (a chunk never spans two
+// steps); an item waits until its pending-upwind counter reaches zero, solves,
+// then decrements the counters of the items that read its psi.  Every
+// dependency of a chunk lies in an earlier chunk, already held by a running
+// block whose own dependencies are earlier still, so the lowest outstanding
+// ticket can always proceed: no assumption on how many blocks are resident.
+// Each wait is bounded by spin_limit polls; a wait that runs out sets the
+// abort flag, after which every block stops taking work and exits.
+template <typename T, int G, int GPL>
+__global__ void __launch_bounds__(kMaxBlock)
+queue_sweep_kernel(const SweepArgs<T> p, const QueueState q)
+{
+    constexpr int LPI = G / GPL;
+    __shared__ int64_t s_chunk;
+    const int64_t slot = threadIdx.x / LPI;
+    const int gl = int(threadIdx.x % LPI);
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int64_t c = int64_t(__hip_atomic_fetch_add(
+                q.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            if (__hip_atomic_load(q.abort, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT) != 0)
+                c = q.n_chunks;
+            s_chunk = c;
+        }
+        __syncthreads();
+        const int64_t c = s_chunk;
+        __syncthreads();
+        if (c >= q.n_chunks)
+            break;
+        const int64_t item = q.chunk_off[c] + slot;
+        const bool active = item < q.chunk_off[c + 1];
+        ItemTopo t{};
+        int32_t *mine = nullptr;
+        bool ok = active;
+        if (active) {
+            t = p.topo[item];
+            mine = q.pending + size_t(t.angle) * size_t(p.nc) + t.cell;
+            int64_t spins = 0;
+            while (__hip_atomic_load(mine, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) > 0) {
+                if (++spins > q.spin_limit) {
+                    __hip_atomic_store(q.abort, 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    ok = false;
+                    break;
+                }
+                if ((spins & 63) == 0 &&
+                    __hip_atomic_load(q.abort, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    ok = false;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        // upwind psi written before the counters reached zero is visible
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (ok)
+            solve_item<T, G, GPL>(p, item, gl, t);
+        // this wavefront's psi stores are visible before any decrement
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (ok && gl == 0) {
+            const int n_up = (t.up[0] >= 0) + (t.up[1] >= 0) +
+                             (t.up[2] >= 0) + (t.up[3] >= 0);
+            // re-armed for the next sweep (nothing else touches it now)
+            __hip_atomic_store(mine, n_up, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            const int4 d = q.down[item];
+            const int32_t dn[4] = {d.x, d.y, d.z, d.w};
+            int32_t *row = q.pending + size_t(t.angle) * size_t(p.nc);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (dn[k] >= 0)
+                    __hip_atomic_fetch_add(row + dn[k], -1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// pos[a][c] = item of (a, c); pending[a][c] = its interior upwind count;
+// down[] cleared.
+__global__ void queue_index_kernel(const ItemTopo *__restrict__ topo,
+                                   int64_t n, int64_t nc,
+                                   int32_t *__restrict__ pos,
+                                   int32_t *__restrict__ pending,
+                                   int32_t *__restrict__ dcount,
+                                   int4 *__restrict__ down)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const ItemTopo t = topo[i];
+    const size_t idx = size_t(t.angle) * size_t(nc) + t.cell;
+    if (pos)
+        pos[idx] = int32_t(i);
+    pending[idx] = (t.up[0] >= 0) + (t.up[1] >= 0) + (t.up[2] >= 0) +
+                   (t.up[3] >= 0);
+    if (dcount) {
+        dcount[i] = 0;
+        down[i] = make_int4(-1, -1, -1, -1);
+    }
+}
+
+// down[item of (a, u)] collects every cell c of angle a that reads u's psi.
+__global__ void queue_down_kernel(const ItemTopo *__restrict__ topo,
+                                  int64_t n, int64_t nc,
+                                  const int32_t *__restrict__ pos,
+                                  int32_t *__restrict__ dcount,
+                                  int4 *__restrict__ down, int *err)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const ItemTopo t = topo[i];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (t.up[k] < 0)
+            continue;
+        const int32_t j = pos[size_t(t.angle) * size_t(nc) + t.up[k]];
+        const int slot = atomicAdd(dcount + j, 1);
+        if (slot < 4)
+            reinterpret_cast<int32_t *>(down + j)[slot] = t.cell;
+        else
+            *err = 1;
     }
 }
 
@@ -300,6 +455,35 @@ hipError_t launch_g(const SweepArgs<T> &args, int gpl, int block,
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- batching
+// Copies one problem's item records into the merged schedule of a batch:
+// item i of step s goes to dst_off[s] + (i - src_off[s]).
+template <typename T>
+__global__ void merge_items_kernel(const ItemGeo<T> *__restrict__ src_geo,
+                                   const ItemTopo *__restrict__ src_topo,
+                                   const int64_t *__restrict__ src_off,
+                                   const int64_t *__restrict__ dst_off,
+                                   int64_t n_steps, int32_t problem,
+                                   ItemGeo<T> *dst_geo, ItemTopo *dst_topo)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= src_off[n_steps])
+        return;
+    int64_t lo = 0, hi = n_steps;   // step s with src_off[s] <= i < src_off[s+1]
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (src_off[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const int64_t d = dst_off[lo] + (i - src_off[lo]);
+    ItemTopo t = src_topo[i];
+    t.problem = problem;
+    dst_topo[d] = t;
+    dst_geo[d] = src_geo[i];
 }
 
 // ---------------------------------------------------------------- residual
@@ -381,10 +565,14 @@ residual_kernel(const ResidualArgs<T> p)
                                       : ge_a12[i] + s + ((P - o[i]) - o[j]);
             r[i] -= m * x[j];
         }
+    // r is a weak-form (integrated) defect; the sweep takes nodal source
+    // densities q with weak form V/20 (I + J) q, so store the density
+    // (20 / V)(I - J / 5) r that reproduces r exactly.
+    const double rs = ((r[0] + r[1]) + (r[2] + r[3])) * 0.2;
     float *dst = p.r + size_t(t.angle) * plane + own + g;
 #pragma unroll
     for (int v = 0; v < 4; v++)
-        dst[v * G] = float(r[v]);
+        dst[v * G] = float((r[v] - rs) / ge_v20);
 }
 
 // ---------------------------------------------------------------- lagging
@@ -653,6 +841,138 @@ hipError_t launch_sweep_step(const SweepArgs<T> &args, int groups,
     }
 }
 
+template <typename T, int G, int GPL>
+hipError_t queue_dispatch(const SweepArgs<T> *args, const QueueState *q,
+                          int block, int64_t grid, hipStream_t stream,
+                          int64_t *resident)
+{
+    if (resident) {
+        int dev = 0, cus = 0, per_cu = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess)
+            e = hipDeviceGetAttribute(
+                &cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu,
+                reinterpret_cast<const void *>(&queue_sweep_kernel<T, G, GPL>),
+                block, 0);
+        *resident = int64_t(std::max(1, per_cu)) * std::max(1, cus);
+        return e;
+    }
+    grid = std::min<int64_t>(grid, std::max<int64_t>(1, q->n_chunks));
+    hipLaunchKernelGGL((queue_sweep_kernel<T, G, GPL>),
+                       dim3(unsigned(std::max<int64_t>(1, grid))), dim3(block),
+                       0, stream, *args, *q);
+    return hipGetLastError();
+}
+
+template <typename T, int G>
+hipError_t queue_g(const SweepArgs<T> *args, const QueueState *q, int gpl,
+                   int block, int64_t grid, hipStream_t stream,
+                   int64_t *resident)
+{
+    if (block < 64 || block > kMaxBlock || block % 64)
+        return hipErrorInvalidConfiguration;
+    switch (gpl) {
+    case 1:
+        return queue_dispatch<T, G, 1>(args, q, block, grid, stream, resident);
+    case 2:
+        if constexpr (G >= 2)
+            return queue_dispatch<T, G, 2>(args, q, block, grid, stream,
+                                           resident);
+        return hipErrorInvalidValue;
+    case 4:
+        if constexpr (G >= 4)
+            return queue_dispatch<T, G, 4>(args, q, block, grid, stream,
+                                           resident);
+        return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t queue_any(const SweepArgs<T> *args, const QueueState *q,
+                     int groups, int gpl, int block, int64_t grid,
+                     hipStream_t stream, int64_t *resident)
+{
+    switch (groups) {
+    case 1: return queue_g<T, 1>(args, q, gpl, block, grid, stream, resident);
+    case 2: return queue_g<T, 2>(args, q, gpl, block, grid, stream, resident);
+    case 4: return queue_g<T, 4>(args, q, gpl, block, grid, stream, resident);
+    case 8: return queue_g<T, 8>(args, q, gpl, block, grid, stream, resident);
+    case 16: return queue_g<T, 16>(args, q, gpl, block, grid, stream, resident);
+    case 32: return queue_g<T, 32>(args, q, gpl, block, grid, stream, resident);
+    case 64: return queue_g<T, 64>(args, q, gpl, block, grid, stream, resident);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t launch_queue_sweep(const SweepArgs<T> &args, const QueueState &q,
+                              int groups, int groups_per_lane, int block,
+                              int64_t grid, hipStream_t stream)
+{
+    if (q.n_chunks <= 0)
+        return hipSuccess;
+    return queue_any<T>(&args, &q, groups, groups_per_lane, block, grid,
+                        stream, nullptr);
+}
+
+template <typename T>
+hipError_t queue_grid(int groups, int groups_per_lane, int block,
+                      int64_t *grid)
+{
+    return queue_any<T>(nullptr, nullptr, groups, groups_per_lane, block, 0,
+                        nullptr, grid);
+}
+
+hipError_t launch_queue_links(const ItemTopo *topo, int64_t n_items,
+                              int64_t nc, int32_t *pos, int32_t *dcount,
+                              int4 *down, int32_t *pending, int *err,
+                              hipStream_t stream)
+{
+    if (n_items <= 0)
+        return hipSuccess;
+    const unsigned grid = unsigned((n_items + 255) / 256);
+    hipLaunchKernelGGL(queue_index_kernel, dim3(grid), dim3(256), 0, stream,
+                       topo, n_items, nc, pos, pending, dcount, down);
+    hipLaunchKernelGGL(queue_down_kernel, dim3(grid), dim3(256), 0, stream,
+                       topo, n_items, nc, pos, dcount, down, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_queue_rearm(const ItemTopo *topo, int64_t n_items,
+                              int64_t nc, int32_t *pending,
+                              hipStream_t stream)
+{
+    if (n_items <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(queue_index_kernel, dim3(unsigned((n_items + 255) / 256)),
+                       dim3(256), 0, stream, topo, n_items, nc,
+                       static_cast<int32_t *>(nullptr), pending,
+                       static_cast<int32_t *>(nullptr),
+                       static_cast<int4 *>(nullptr));
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_merge_items(const ItemGeo<T> *src_geo,
+                              const ItemTopo *src_topo,
+                              const int64_t *src_off, const int64_t *dst_off,
+                              int64_t n_steps, int64_t n_items,
+                              int32_t problem, ItemGeo<T> *dst_geo,
+                              ItemTopo *dst_topo, hipStream_t stream)
+{
+    if (n_items <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL((merge_items_kernel<T>),
+                       dim3(unsigned((n_items + 255) / 256)), dim3(256), 0,
+                       stream, src_geo, src_topo, src_off, dst_off, n_steps,
+                       problem, dst_geo, dst_topo);
+    return hipGetLastError();
+}
+
 template <typename T, int G>
 hipError_t launch_residual_g(const ResidualArgs<T> &args, hipStream_t stream)
 {
@@ -802,6 +1122,14 @@ hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t stream)
                                              int, hipStream_t);               \
     template hipError_t launch_residual<T>(const ResidualArgs<T> &, int,      \
                                            hipStream_t);                      \
+    template hipError_t launch_queue_sweep<T>(const SweepArgs<T> &,           \
+                                              const QueueState &, int, int,   \
+                                              int, int64_t, hipStream_t);     \
+    template hipError_t queue_grid<T>(int, int, int, int64_t *);              \
+    template hipError_t launch_merge_items<T>(                                \
+        const ItemGeo<T> *, const ItemTopo *, const int64_t *,                \
+        const int64_t *, int64_t, int64_t, int32_t, ItemGeo<T> *,             \
+        ItemTopo *, hipStream_t);                                             \
     template hipError_t launch_lag_gather<T>(const int64_t *, const int32_t *,\
                                              int64_t, int64_t, int,           \
                                              const uint32_t *, const T *,     \

@@ -33,13 +33,26 @@ struct alignas(16) ItemGeo {
 // parallel or on the vacuum boundary, or -2 - i for lagged face i (its
 // values come from the pre-sweep snapshot lag_buf[i]).  meta holds, for each
 // face k, the neighbour vertex facing cell vertex j (j != k, ascending) in
-// bits 6k + 2*slot.
+// bits 6k + 2*slot.  problem indexes the problem table of a batched launch
+// (0 for a single problem).
 struct alignas(16) ItemTopo {
     int32_t cell;
     int32_t angle;
     int32_t up[4];
     uint32_t meta;
-    int32_t pad;
+    int32_t problem;
+};
+
+// Per-problem pointers of a batched launch (several independent problems
+// with the same group count and precision swept by one launch per step).
+template <typename T>
+struct ProblemDesc {
+    int64_t nc;
+    int sigma_mode;
+    const T *sigma;
+    const T *src_iso;
+    T *psi;
+    const T *lag_buf;
 };
 
 enum SigmaMode { kSigmaCellGroup = 0, kSigmaGroup = 1, kSigmaCell = 2 };
@@ -57,11 +70,16 @@ struct SweepArgs {
     const T *src_ang_t;        // [na][nc][4][G] same, in T, or null
     T *psi;                    // [na][nc][4][G]
     const T *lag_buf;          // [n_lag][4][G] upwind values held pre-sweep
+    // batched launch: per-problem pointers replace nc, sigma, src_iso, psi
+    // and lag_buf (src_ang and src_ang_t must be null)
+    const ProblemDesc<T> *problems;
 };
 
 // Defect of the coupled transport equations for a stored angular flux,
 // r = q + S phi - L psi, evaluated in FP64 from FP64 item records (lagged
-// faces read the upwind cell's current psi: the operator is the full one).
+// faces read the upwind cell's current psi: the operator is the full one),
+// stored as the nodal source density whose weak form is r, so that a sweep
+// with it as source gives L^-1 r.
 template <typename T>
 struct ResidualArgs {
     const ItemGeo<double> *geo;
@@ -92,11 +110,54 @@ hipError_t launch_build_items(const uint32_t *work, int64_t n, int cell_bits,
                               ItemGeo<T> *out_geo, ItemTopo *out_topo,
                               int *err, hipStream_t stream);
 
+// Copies one problem's step-ordered records into a batch's merged schedule
+// (src_off / dst_off: [n_steps + 1] step offsets on the device).
+template <typename T>
+hipError_t launch_merge_items(const ItemGeo<T> *src_geo,
+                              const ItemTopo *src_topo,
+                              const int64_t *src_off, const int64_t *dst_off,
+                              int64_t n_steps, int64_t n_items,
+                              int32_t problem, ItemGeo<T> *dst_geo,
+                              ItemTopo *dst_topo, hipStream_t stream);
+
 // Sweep of one step: every item's cell solve for all groups.
 template <typename T>
 hipError_t launch_sweep_step(const SweepArgs<T> &args, int groups,
                              int groups_per_lane, int block,
                              hipStream_t stream);
+
+// Ready-queue form of a sweep: one persistent launch over the step-ordered
+// items, split into chunks that never span two steps.
+struct QueueState {
+    const int64_t *chunk_off;      // [n_chunks + 1] item ranges
+    int64_t n_chunks;
+    int32_t *pending;              // [na][nc] upwind cells not yet solved
+    const int4 *down;              // [n_items] cells reading the item's psi
+    unsigned long long *ticket;    // next chunk to hand out (0 at the start)
+    int32_t *abort;                // set when a wait exceeds spin_limit
+    int64_t spin_limit;            // polls per wait before giving up
+};
+
+// grid: blocks to launch (queue_grid gives the number resident at once).
+template <typename T>
+hipError_t launch_queue_sweep(const SweepArgs<T> &args, const QueueState &q,
+                              int groups, int groups_per_lane, int block,
+                              int64_t grid, hipStream_t stream);
+// Blocks of the queue kernel resident on the current device at once.
+template <typename T>
+hipError_t queue_grid(int groups, int groups_per_lane, int block,
+                      int64_t *grid);
+
+// Builds down[] and pending[] from the item records (pos and dcount: scratch
+// of na * nc and n_items int32); *err is set if an item had > 4 readers.
+hipError_t launch_queue_links(const ItemTopo *topo, int64_t n_items,
+                              int64_t nc, int32_t *pos, int32_t *dcount,
+                              int4 *down, int32_t *pending, int *err,
+                              hipStream_t stream);
+// Resets pending[] to the upwind counts (after an aborted sweep).
+hipError_t launch_queue_rearm(const ItemTopo *topo, int64_t n_items,
+                              int64_t nc, int32_t *pending,
+                              hipStream_t stream);
 
 // lag_buf[i][j][g] = psi[a_i][u_i][vertex of u_i facing c_i's vertex j][g]
 template <typename T>

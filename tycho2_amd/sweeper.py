@@ -21,6 +21,9 @@ from . import build as _build
 FP64 = 0
 FP32 = 1
 GAUSS_NOPIVOT = 1
+ENGINE_LEVELS = 0
+ENGINE_QUEUE = 1
+ERR_TIMEOUT = -7
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int64)
@@ -44,6 +47,8 @@ SIGNATURES = {
     "t2_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
                                      ctypes.c_int]),
     "t2_set_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "t2_set_engine": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.c_int64]),
     "t2_solve_refined": (ctypes.c_int, [
         ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
         ctypes.c_int64, ctypes.c_void_p, _dp]),
@@ -66,6 +71,16 @@ SIGNATURES = {
     "t2_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "t2_get_stream": (ctypes.c_int, [ctypes.c_void_p,
                                      ctypes.POINTER(ctypes.c_void_p)]),
+    "t2_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.c_int]),
+    "t2_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "t2_batch_sweep_step": (ctypes.c_int, [ctypes.c_void_p]),
+    "t2_batch_sweep_only": (ctypes.c_int, [ctypes.c_void_p]),
+    "t2_batch_info": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip]),
+    "t2_batch_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "t2_batch_get_stream": (ctypes.c_int, [ctypes.c_void_p,
+                                           ctypes.POINTER(ctypes.c_void_p)]),
 }
 
 
@@ -220,6 +235,12 @@ class SweeperHIP:
     def set_schedule(self, angle_batches: int) -> None:
         _check(lib().t2_set_schedule(self.handle, int(angle_batches)))
 
+    def set_engine(self, engine: int, spin_limit: int = 0) -> None:
+        """ENGINE_LEVELS (per-step launches in a hipGraph) or ENGINE_QUEUE
+        (one persistent ready-queue launch); spin_limit bounds each queue
+        wait in polls (0: library default)."""
+        _check(lib().t2_set_engine(self.handle, int(engine), int(spin_limit)))
+
     # -- SweeperAbstract::sweep
     def sweep(self, psi: np.ndarray, source: np.ndarray,
               zero_psi_bound: bool = False) -> np.ndarray:
@@ -306,3 +327,63 @@ class SweeperHIP:
         out = np.empty((self.nc, self.na, 4, self.ng))
         _check(lib().t2_get_psi(self.handle, _ptr(out)))
         return out
+
+
+class SweepBatch:
+    """Several sweepers swept together, one launch per merged step.
+
+    Members keep their own psi, phi and sources; each batch sweep gives every
+    member exactly what its own sweep would.  Members must stay alive and
+    keep their schedule, launch shape and source kind while the batch is
+    used (the library refuses a batch whose members changed)."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(
+            *[m.handle.value for m in self.members])
+        h = ctypes.c_void_p()
+        _check(lib().t2_batch_create(ctypes.byref(h), arr, len(self.members)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().t2_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise T2Error(-4, "batch is closed")
+        return self._h
+
+    def sweep_step(self) -> None:
+        _check(lib().t2_batch_sweep_step(self.handle))
+
+    def sweep_only(self) -> None:
+        _check(lib().t2_batch_sweep_only(self.handle))
+
+    def synchronize(self) -> None:
+        _check(lib().t2_batch_synchronize(self.handle))
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        _check(lib().t2_batch_get_stream(self.handle, ctypes.byref(s)))
+        return s.value or 0
+
+    def info(self) -> dict:
+        n_steps, n_items = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().t2_batch_info(self.handle, ctypes.byref(n_steps),
+                                   ctypes.byref(n_items)))
+        return {"n_steps": n_steps.value, "n_items": n_items.value}
