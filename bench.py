@@ -60,9 +60,10 @@ def path_leg(p, precision, steps, warmup, ev, tune, solve):
     t0 = time.time()
     s = SweeperHIP.from_problem(p, precision)
     out = dict(setup_s=round(time.time() - t0, 2))
+    log(f"precision {precision}: handle ready in {out['setup_s']}s")
     try:
         if tune:
-            best, _ = autotune.tune(s, reps=2)
+            best, _ = autotune.tune(s, reps=2, log=log)
             out["tuned"] = best
             log(f"autotune: {best}")
         info = s.info()
@@ -71,10 +72,13 @@ def path_leg(p, precision, steps, warmup, ev, tune, solve):
         s.synchronize()
         out["ms_per_step"] = time_region(ev, s, s.sweep_step, steps) / steps
         out["ms_per_sweep"] = time_region(ev, s, s.sweep_only, steps) / steps
+        log(f"precision {precision}: {out['ms_per_step']:.2f} ms/step, "
+            f"{out['ms_per_sweep']:.2f} ms/sweep")
         out["n_steps"] = info["n_steps"]
         out["psi_gb"] = info["psi_bytes"] / 1e9
         if solve:
             cfg = p.solver
+            sweeps0 = s.info()["sweeps"]
             if precision == FP32:
                 holder = {}
 
@@ -86,7 +90,6 @@ def path_leg(p, precision, steps, warmup, ev, tune, solve):
                 out["solve_ms"] = ms
                 out["refine"] = holder["info"]
                 out["phi1"] = holder["phi1"]
-                out["sweeps"] = holder["info"]["sweeps"]
             else:
                 holder = {}
 
@@ -97,9 +100,11 @@ def path_leg(p, precision, steps, warmup, ev, tune, solve):
                 ms = time_region(ev, s, run, 1)
                 out["solve_ms"] = ms
                 out["iters"] = len(holder["hist"])
-                out["sweeps"] = len(holder["hist"]) + 3
                 out["residual"] = float(holder["hist"][-1]) \
                     if len(holder["hist"]) else 0.0
+            out["sweeps"] = s.info()["sweeps"] - sweeps0
+            log(f"precision {precision}: solve {out['solve_ms']:.1f} ms, "
+                f"{out['sweeps']} sweeps")
             out["phi"] = s.phi()
             _ = np
     finally:
@@ -202,7 +207,16 @@ def main() -> int:
     solves = p.solves_per_sweep
     b64, b32 = alg_bytes_per_solve(8, n, p.na), alg_bytes_per_solve(4, n, p.na)
 
-    def roof(r, width, bytes_per_solve, kernel):
+    def roof(r, width, bytes_per_solve):
+        t = r.get("tuned") or {}
+        gpl = t.get("groups_per_lane", 1)
+        ty = "double" if width == 8 else "float"
+        if t.get("engine", 0) == 1:
+            kernel = f"queue_sweep_kernel<{ty},{p.ng},{gpl}>"
+            launches = 1
+        else:
+            kernel = f"sweep_step_kernel<{ty},{p.ng},{gpl}>"
+            launches = r["n_steps"]
         ach = solves * bytes_per_solve / (r["ms_per_sweep"] * 1e-3) / 1e9
         return {"bound": "hbm", "kernel": kernel, "achieved": round(ach, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -211,22 +225,22 @@ def main() -> int:
                                                  f"fp{8 * width}"),
                 "alg_bytes_per_solve": round(bytes_per_solve, 3),
                 "alg_bytes_per_launch": round(solves * bytes_per_solve /
-                                              r["n_steps"]),
-                "avg_launch_ms": round(r["ms_per_sweep"] / r["n_steps"], 5),
-                "launches_per_sweep": r["n_steps"]}
+                                              launches),
+                "avg_launch_ms": round(r["ms_per_sweep"] / launches, 5),
+                "launches_per_sweep": launches}
 
     plain = {"solves_per_s_step": round(solves / (r64["ms_per_step"] * 1e-3)),
              "solves_per_s_sweep": round(solves /
                                          (r64["ms_per_sweep"] * 1e-3)),
              "ms_per_sweep": round(r64["ms_per_sweep"], 3),
              "tuned": r64.get("tuned"),
-             "roofline": roof(r64, 8, b64, "sweep_step_kernel<double,16,1>")}
+             "roofline": roof(r64, 8, b64)}
     refined = {"solves_per_s_step": round(solves / (r32["ms_per_step"] * 1e-3)),
                "solves_per_s_sweep": round(solves /
                                            (r32["ms_per_sweep"] * 1e-3)),
                "ms_per_sweep": round(r32["ms_per_sweep"], 3),
                "tuned": r32.get("tuned"),
-               "roofline": roof(r32, 4, b32, "sweep_step_kernel<float,16,1>")}
+               "roofline": roof(r32, 4, b32)}
     value = plain["solves_per_s_step"]
     metric_path = "FP64 plain, source-iteration step"
     ms_per_step = r64["ms_per_step"]

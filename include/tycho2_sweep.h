@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define T2_ABI_VERSION 1
+#define T2_ABI_VERSION 2
 
 /* status codes */
 #define T2_OK 0
@@ -66,6 +66,13 @@ typedef struct {
     int64_t device_bytes;     /* all device memory held by the handle      */
     int32_t precision;        /* T2_FP64 / T2_FP32                         */
     int32_t gauss;            /* T2_GAUSS_*                                */
+    int64_t sweeps;           /* full sweeps issued by the handle so far   */
+    int32_t engine;           /* T2_ENGINE_*                               */
+    int32_t angle_batches;    /* schedule knob (t2_set_schedule)           */
+    int32_t items_per_block;  /* launch knobs (t2_set_launch)              */
+    int32_t groups_per_lane;
+    int32_t chunk_rounds;     /* queue knob (t2_set_engine)                */
+    int32_t reserved;
 } t2_info;
 
 /* Message of the last failure on this thread ("" if none). */
@@ -127,13 +134,16 @@ int t2_set_schedule(t2_sweeper *s, int angle_batches);
  *                     schedule order from a device ticket counter, each item
  *                     waits on its atomic pending-upwind counter, solves, and
  *                     decrements the counters of the cells that read it.
- * Results are bitwise the same.  spin_limit bounds every queue wait (polls;
- * 0 = default); a wait that runs out abandons the sweep, and the next
- * synchronising call on the handle returns T2_ERR_TIMEOUT (the counters are
- * re-armed, so later sweeps are valid). */
+ * Results are bitwise the same.  chunk_rounds: items per lane slot in one
+ * queue chunk (a chunk is chunk_rounds x items_per_block items; 0 = 4).
+ * spin_limit bounds every queue wait (polls; 0 = default); a wait that runs
+ * out abandons the sweep, and the next synchronising call on the handle
+ * returns T2_ERR_TIMEOUT (the counters are re-armed, so later sweeps are
+ * valid). */
 #define T2_ENGINE_LEVELS 0
 #define T2_ENGINE_QUEUE 1
-int t2_set_engine(t2_sweeper *s, int engine, int64_t spin_limit);
+int t2_set_engine(t2_sweeper *s, int engine, int chunk_rounds,
+                  int64_t spin_limit);
 
 /* ------------------------------------------------------------ drop-in sweep
  * SweeperAbstract::sweep(psi, source, zeroPsiBound) on host arrays in the
@@ -198,18 +208,31 @@ int t2_sweep_step(t2_sweeper *s);
 int t2_sweep_only(t2_sweeper *s);
 
 /* ------------------------------------------------------------ batches
- * Several independent problems (handles on one device with the same group
- * count and precision, isotropic sources) swept together: their schedules
- * are merged step by step, so one kernel launch per merged step covers every
- * member (small problems no longer pay a launch per level each).  Members
- * keep their own psi, sources and phi; results equal the members' own
- * sweeps bit for bit.  No reference counterpart (one problem per run). */
+ * Several independent problems (handles on one device with one precision and
+ * isotropic sources; meshes, quadratures and group counts may differ) swept
+ * together.  Members keep their own psi, sources and phi; each batch sweep
+ * gives every member exactly what its own sweep would, bit for bit.
+ *   T2_ENGINE_LEVELS  members with the same group count share one merged
+ *                     step schedule (one launch per merged step and group
+ *                     count, replayed from a hipGraph); the merged records
+ *                     are a copy of the members' records;
+ *   T2_ENGINE_QUEUE   one persistent launch for all members: chunks of the
+ *                     members' items interleaved step by step behind one
+ *                     ticket counter, each waiting on its own problem's
+ *                     pending-upwind counters (no copy of the records).
+ * Members must outlive the batch; a member whose schedule or source kind
+ * changes afterwards makes batch calls fail with T2_ERR_STATE.  Wait limits
+ * behave as for one handle (t2_batch_synchronize reports T2_ERR_TIMEOUT).
+ * No reference counterpart (the reference runs one problem per process). */
 typedef struct t2_batch t2_batch;
 int t2_batch_create(t2_batch **out, t2_sweeper *const *members, int count);
 int t2_batch_destroy(t2_batch *b);
-/* t2_sweep_step of every member with one merged sweep */
+int t2_batch_set_engine(t2_batch *b, int engine, int chunk_rounds,
+                        int64_t spin_limit);
+/* t2_sweep_step / t2_sweep_only of every member in one batched sweep */
 int t2_batch_sweep_step(t2_batch *b);
 int t2_batch_sweep_only(t2_batch *b);
+/* longest member schedule (steps) and total (angle, cell) items */
 int t2_batch_info(const t2_batch *b, int64_t *n_steps, int64_t *n_items);
 int t2_batch_synchronize(t2_batch *b);
 int t2_batch_get_stream(const t2_batch *b, void **stream);

@@ -48,10 +48,13 @@ def test_driver_usage_and_errors(tmp_path):
     r = run(GOLD_DECK.format(out=out, extra="").replace("snOrder         8",
                                                         ""), tmp_path)
     assert r.returncode == 1 and "snOrder" in r.stderr
+    r = run(GOLD_DECK.format(out=out, extra="SweepEngine Fast"), tmp_path)
+    assert r.returncode == 1 and "SweepEngine" in r.stderr
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("extra", ["DeviceLoop false", "DeviceLoop true"])
+@pytest.mark.parametrize("extra", ["DeviceLoop false", "DeviceLoop true",
+                                   "DeviceLoop true\nSweepEngine Queue"])
 def test_driver_reproduces_gold(tmp_path, gold, extra):
     print(f"driver gold {extra}", flush=True)
     out = str(tmp_path / "out.psi")

@@ -30,11 +30,12 @@ def problem(n, sn, ng, seed, jitter=0.15):
     return sm, conn, omega, wt, st, ss, q
 
 
-def run_steps(args, precision, engine, steps=3, batches=1, launch=None):
+def run_steps(args, precision, engine, steps=3, batches=1, launch=None,
+              rounds=0):
     sm, conn, omega, wt, st, ss, q = args
     ng = q.shape[2]
     with SweeperHIP(sm, conn, omega, wt, ng, st, ss, precision) as s:
-        s.set_engine(engine, SPIN)
+        s.set_engine(engine, SPIN, rounds)
         if batches != 1:
             s.set_schedule(batches)
         if launch:
@@ -56,13 +57,17 @@ def test_queue_matches_levels_bitwise(precision, ng):
     assert np.array_equal(ref[1], got[1])
 
 
-@pytest.mark.parametrize("batches,launch", [(2, None), (1, (8, 2)),
-                                            (4, (4, 1))])
-def test_queue_schedules_and_shapes_bitwise(batches, launch):
-    print(f"queue batches={batches} launch={launch}", flush=True)
+@pytest.mark.parametrize("batches,launch,rounds", [(2, None, 1),
+                                                   (1, (8, 2), 4),
+                                                   (4, (4, 1), 16),
+                                                   (1, None, 64)])
+def test_queue_schedules_and_shapes_bitwise(batches, launch, rounds):
+    print(f"queue batches={batches} launch={launch} rounds={rounds}",
+          flush=True)
     args = problem(4, 4, 16, seed=52)
     ref = run_steps(args, FP64, ENGINE_LEVELS)
-    got = run_steps(args, FP64, ENGINE_QUEUE, batches=batches, launch=launch)
+    got = run_steps(args, FP64, ENGINE_QUEUE, batches=batches, launch=launch,
+                    rounds=rounds)
     assert np.array_equal(ref[0], got[0])
     assert np.array_equal(ref[1], got[1])
 

@@ -257,12 +257,12 @@ def test_refined_solve_reaches_1e10(ratio, bar):
     assert e1 > 100 * e2
 
 
-def batch_members(precision, ng=4):
-    """Three problems of different mesh, quadrature and cross sections with
-    one group count; the third has lagged faces."""
+def batch_members(precision, groups=(4, 4, 4)):
+    """Three problems of different mesh, quadrature and cross sections; the
+    third has lagged faces."""
     specs = [(3, 4, 0.15, 31), (5, 2, 0.1, 32), (3, 2, 0.2, 3)]
     out = []
-    for n, sn, jit, seed in specs:
+    for (n, sn, jit, seed), ng in zip(specs, groups):
         sm = meshmod.kuhn_cube(n, jit, seed=seed)
         conn = meshmod.tycho_connectivity(sm)
         omega, wt = (orc.quadrature(sn) if sn == 2 else
@@ -277,20 +277,24 @@ def batch_members(precision, ng=4):
     return out
 
 
-@pytest.mark.parametrize("precision", [FP64, FP32])
-def test_batch_sweep_bitwise(precision):
-    """A merged-schedule batch gives every member what its own source
-    iteration steps give, bit for bit, and the FP64 batch matches the
-    oracle's sweeps."""
+@pytest.mark.parametrize("precision,engine,groups", [
+    (FP64, 0, (4, 4, 4)), (FP32, 0, (4, 4, 4)), (FP64, 0, (1, 16, 4)),
+    (FP64, 1, (1, 16, 4)), (FP32, 1, (32, 2, 4)), (FP64, 1, (64, 8, 1))])
+def test_batch_sweep_bitwise(precision, engine, groups):
+    """A batch (merged per-step launches, or one persistent ready-queue
+    launch for all members) gives every member what its own source
+    iteration steps give, bit for bit."""
     from tycho2_amd.sweeper import SweepBatch
-    print(f"batch sweep precision={precision}", flush=True)
-    own = batch_members(precision)
-    mem = batch_members(precision)
+    print(f"batch sweep precision={precision} engine={engine} "
+          f"groups={groups}", flush=True)
+    own = batch_members(precision, groups)
+    mem = batch_members(precision, groups)
     assert mem[2].info()["n_lagged"] > 0
     for s in own:
         for _ in range(3):
             s.sweep_step()
     with SweepBatch(mem) as b:
+        b.set_engine(engine, 1 << 20)
         inf = b.info()
         assert inf["n_steps"] == max(s.info()["n_steps"] for s in mem)
         for _ in range(3):

@@ -6,14 +6,18 @@ stream) over
     after the other (smaller per-step working set, more steps),
   * cells per block and groups per lane -- the launch shape of the sweep
     kernels,
+  * the engine -- per-step launches replayed from a hipGraph, or one
+    persistent ready-queue launch,
 keeps the fastest setting on the handle and returns the measurements.
 Results do not depend on any of these knobs (bitwise, tested).
 """
 from __future__ import annotations
 
+from .sweeper import ENGINE_LEVELS, ENGINE_QUEUE
 from .timing import HipEvents, time_region
 
 BATCHES = (1, 2, 4, 8, 16)
+CHUNK_ROUNDS = (2, 4, 8)          # ready-queue chunk size, in rounds
 BLOCKS = (256, 128, 64)          # threads per block
 GROUPS_PER_LANE = (1, 2)
 
@@ -25,13 +29,16 @@ def _time_sweep(ev, s, reps):
 
 
 def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
-         groups_per_lane=GROUPS_PER_LANE, verbose: bool = False):
+         groups_per_lane=GROUPS_PER_LANE,
+         engines=(ENGINE_LEVELS, ENGINE_QUEUE), log=None):
     """Tune handle `s` in place; returns (best, trials) where best is a dict
-    {batches, block, groups_per_lane, ms} and trials lists every timing."""
+    {engine, batches, block, groups_per_lane, ms} and trials lists every
+    timing; log (a callable) receives one line per trial."""
     ev = HipEvents()
     ng = s.ng
     trials = []
     best = None
+    s.set_engine(ENGINE_LEVELS)
     # schedule first (it rebuilds the records), with the default launch
     for nb in batches:
         if nb > s.na:
@@ -39,10 +46,11 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
         s.set_schedule(nb)
         s.set_launch(256 // ng if ng <= 256 else 1, 1)
         ms = _time_sweep(ev, s, reps)
-        t = dict(batches=nb, block=256, groups_per_lane=1, ms=ms)
+        t = dict(engine=ENGINE_LEVELS, batches=nb, block=256,
+                 groups_per_lane=1, ms=ms)
         trials.append(t)
-        if verbose:
-            print(t, flush=True)
+        if log:
+            log(f"tune {t}")
         if best is None or ms < best["ms"]:
             best = t
     s.set_schedule(best["batches"])
@@ -55,13 +63,33 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
                 continue
             s.set_launch(block // lanes, gpl)
             ms = _time_sweep(ev, s, reps)
-            t = dict(batches=best["batches"], block=block,
-                     groups_per_lane=gpl, ms=ms)
+            t = dict(engine=ENGINE_LEVELS, batches=best["batches"],
+                     block=block, groups_per_lane=gpl, ms=ms)
             trials.append(t)
-            if verbose:
-                print(t, flush=True)
+            if log:
+                log(f"tune {t}")
             if ms < best["ms"]:
                 best = t
     lanes = ng // best["groups_per_lane"]
     s.set_launch(best["block"] // lanes, best["groups_per_lane"])
+    # the persistent engine at the best shape, over the schedules it may
+    # prefer (fewer steps matter less to it)
+    if ENGINE_QUEUE in engines:
+        level_best = best
+        for rounds in CHUNK_ROUNDS:
+            s.set_engine(ENGINE_QUEUE, chunk_rounds=rounds)
+            for nb in sorted({1, level_best["batches"]}):
+                s.set_schedule(nb)
+                ms = _time_sweep(ev, s, reps)
+                t = dict(engine=ENGINE_QUEUE, batches=nb,
+                         block=level_best["block"],
+                         groups_per_lane=level_best["groups_per_lane"],
+                         chunk_rounds=rounds, ms=ms)
+                trials.append(t)
+                if log:
+                    log(f"tune {t}")
+                if ms < best["ms"]:
+                    best = t
+    s.set_engine(best["engine"], chunk_rounds=best.get("chunk_rounds", 0))
+    s.set_schedule(best["batches"])
     return best, trials

@@ -85,7 +85,9 @@ struct t2_sweeper {
     // ready-queue engine, built on first use for the schedule and launch
     // shape (n_chunks < 0: not built)
     int engine = T2_ENGINE_LEVELS;
+    int chunk_rounds = 4;
     int64_t spin_limit = 0;
+    int64_t n_sweeps = 0;            // full sweeps issued
     int64_t n_chunks = -1, queue_grid = 0;
     int64_t *d_chunk_off = nullptr;
     int32_t *d_pending = nullptr;    // [na][nc]
@@ -249,7 +251,8 @@ int build_queue(t2_sweeper *s)
     const int64_t n_items = nc * na;
     if (n_items >= (int64_t(1) << 31))
         return fail(T2_ERR_ARG, "ready queue: more than 2^31 items");
-    const int64_t per_chunk = s->block / (s->ng / s->gpl);
+    const int64_t per_chunk =
+        int64_t(s->block / (s->ng / s->gpl)) * s->chunk_rounds;
     std::vector<int64_t> off(1, 0);
     for (int64_t st = 0; st < s->sched.n_steps; st++)
         for (int64_t i = s->sched.step_off[st]; i < s->sched.step_off[st + 1];
@@ -340,6 +343,7 @@ int run_sweep(t2_sweeper *s, const double *ang, const void *ang_t = nullptr)
 {
     if (s->engine == T2_ENGINE_QUEUE && s->n_chunks < 0)
         T2_TRY(build_queue(s));
+    s->n_sweeps++;
     if (!s->use_graph)
         return enqueue_sweep_any(s, ang, ang_t);
     hipGraphExec_t exec = nullptr;
@@ -848,6 +852,13 @@ int t2_sweeper_info(const t2_sweeper *s, t2_info *info)
     info->device_bytes = int64_t(s->bytes);
     info->precision = s->precision;
     info->gauss = s->gauss;
+    info->sweeps = s->n_sweeps;
+    info->engine = s->engine;
+    info->angle_batches = s->batches;
+    info->items_per_block = int32_t(s->block / (s->ng / s->gpl));
+    info->groups_per_lane = s->gpl;
+    info->chunk_rounds = s->chunk_rounds;
+    info->reserved = 0;
     return T2_OK;
 }
 
@@ -1192,7 +1203,8 @@ int t2_synchronize(t2_sweeper *s)
     return check_abort(s);
 }
 
-int t2_set_engine(t2_sweeper *s, int engine, int64_t spin_limit)
+int t2_set_engine(t2_sweeper *s, int engine, int chunk_rounds,
+                  int64_t spin_limit)
 {
     t2::clear_error();
     T2_TRY(check(s));
@@ -1201,10 +1213,16 @@ int t2_set_engine(t2_sweeper *s, int engine, int64_t spin_limit)
                                 "T2_ENGINE_QUEUE");
     if (spin_limit < 0)
         return fail(T2_ERR_ARG, "spin_limit < 0");
+    if (chunk_rounds < 0 || chunk_rounds > 256)
+        return fail(T2_ERR_ARG, "chunk_rounds must be in [0, 256]");
+    const int rounds = chunk_rounds ? chunk_rounds : 4;
     T2_HIP(hipStreamSynchronize(s->stream));
     if (engine != s->engine || spin_limit != s->spin_limit)
         drop_graphs(s);
+    if (rounds != s->chunk_rounds)
+        free_queue(s);
     s->engine = engine;
+    s->chunk_rounds = rounds;
     s->spin_limit = spin_limit;
     return T2_OK;
 }
@@ -1250,21 +1268,42 @@ int t2_get_stream(const t2_sweeper *s, void **stream)
 }  // extern "C"
 
 // ====================================================================== batch
-struct t2_batch {
-    std::vector<t2_sweeper *> members;
-    int device = 0;
-    int precision = T2_FP64;
-    int64_t ng = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t order = nullptr;          // orders member and batch streams
+namespace {
+
+// Members with one group count, swept by one merged level schedule.
+struct LevelGroup {
+    int ng = 0;
+    std::vector<int> idx;                // member indices
     std::vector<int64_t> step_off;       // merged schedule
     int64_t n_items = 0;
     void *d_geo = nullptr;
     t2::ItemTopo *d_topo = nullptr;
     void *d_desc = nullptr;
-    size_t geo_bytes = 0, desc_bytes = 0;
+};
+
+}  // namespace
+
+struct t2_batch {
+    std::vector<t2_sweeper *> members;
+    int device = 0;
+    int precision = T2_FP64;
+    hipStream_t stream = nullptr;
+    hipEvent_t order = nullptr;          // orders member and batch streams
+    int engine = T2_ENGINE_LEVELS;
+    int chunk_rounds = 4;
+    int64_t spin_limit = 0;
+    int64_t n_steps = 0, n_items = 0;
+    // level form (built on first use)
+    bool levels_built = false;
+    std::vector<LevelGroup> groups;
+    // queue form (built on first use)
+    int64_t n_chunks = -1, grid = 0;
+    void *d_problems = nullptr;
+    t2::BatchChunk *d_chunks = nullptr;
+    unsigned long long *d_ticket = nullptr;
+    int32_t *d_abort = nullptr;
     hipGraphExec_t graph = nullptr;
-    std::vector<std::vector<const void *>> snap;  // members' buffers at create
+    std::vector<std::vector<const void *>> snap;  // members' buffers
 };
 
 namespace {
@@ -1272,14 +1311,19 @@ namespace {
 std::vector<const void *> member_buffers(const t2_sweeper *m)
 {
     return {m->d_item_geo, m->d_item_topo, m->d_psi, m->d_src, m->d_lag_buf,
-            m->d_lag_key, m->d_sigt, m->d_qang,
-            reinterpret_cast<const void *>(intptr_t(m->sched.n_steps)),
-            reinterpret_cast<const void *>(intptr_t(m->block)),
-            reinterpret_cast<const void *>(intptr_t(m->gpl))};
+            m->d_lag_key, m->d_sigt, m->d_qang, m->d_pending, m->d_down,
+            reinterpret_cast<const void *>(intptr_t(m->sched.n_steps))};
 }
 
-// A member whose schedule, launch shape, buffers or source kind changed since
-// the batch was made would be swept with stale records.
+void batch_snapshot(t2_batch *b)
+{
+    b->snap.clear();
+    for (auto *m : b->members)
+        b->snap.push_back(member_buffers(m));
+}
+
+// A member whose schedule, buffers or source kind changed since the batch
+// last built its tables would be swept with stale records.
 int batch_check(const t2_batch *b)
 {
     if (!b)
@@ -1288,8 +1332,8 @@ int batch_check(const t2_batch *b)
         if (member_buffers(b->members[i]) != b->snap[i])
             return fail(T2_ERR_STATE,
                         "batch member " + std::to_string(i) +
-                            " changed its schedule, launch shape or source "
-                            "since t2_batch_create");
+                            " changed its schedule or source since the "
+                            "batch was built");
     return T2_OK;
 }
 
@@ -1300,36 +1344,71 @@ int batch_fail_cleanup(t2_batch *b, int rc)
     return fail(rc, msg);
 }
 
-template <typename T>
-int batch_build(t2_batch *b)
+void free_levels(t2_batch *b)
 {
-    const int nm = int(b->members.size());
+    for (auto &g : b->groups) {
+        if (g.d_geo)
+            (void)hipFree(g.d_geo);
+        if (g.d_topo)
+            (void)hipFree(g.d_topo);
+        if (g.d_desc)
+            (void)hipFree(g.d_desc);
+    }
+    b->groups.clear();
+    b->levels_built = false;
+}
+
+void free_batch_queue(t2_batch *b)
+{
+    if (b->d_problems)
+        (void)hipFree(b->d_problems);
+    if (b->d_chunks)
+        (void)hipFree(b->d_chunks);
+    b->d_problems = nullptr;
+    b->d_chunks = nullptr;
+    b->n_chunks = -1;
+}
+
+void drop_batch_graph(t2_batch *b)
+{
+    if (b->graph)
+        (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+}
+
+// Level form: members grouped by group count; each group's schedules merged
+// step by step into one record stream with a problem table.
+template <typename T>
+int build_level_group(t2_batch *b, LevelGroup &g)
+{
     int64_t n_steps = 0;
-    for (auto *m : b->members)
-        n_steps = std::max(n_steps, m->sched.n_steps);
+    for (int i : g.idx)
+        n_steps = std::max(n_steps, b->members[i]->sched.n_steps);
     std::vector<int64_t> count(n_steps, 0);
-    for (auto *m : b->members)
+    for (int i : g.idx) {
+        const t2_sweeper *m = b->members[i];
         for (int64_t st = 0; st < m->sched.n_steps; st++)
             count[st] += m->sched.step_off[st + 1] - m->sched.step_off[st];
-    b->step_off.assign(n_steps + 1, 0);
+    }
+    g.step_off.assign(n_steps + 1, 0);
     for (int64_t st = 0; st < n_steps; st++)
-        b->step_off[st + 1] = b->step_off[st] + count[st];
-    b->n_items = b->step_off[n_steps];
-    b->geo_bytes = sizeof(t2::ItemGeo<T>) * size_t(b->n_items);
-    T2_HIP(hipMalloc(&b->d_geo, b->geo_bytes ? b->geo_bytes : 16));
-    T2_HIP(hipMalloc(reinterpret_cast<void **>(&b->d_topo),
-                     sizeof(t2::ItemTopo) * size_t(std::max<int64_t>(1, b->n_items))));
+        g.step_off[st + 1] = g.step_off[st] + count[st];
+    g.n_items = g.step_off[n_steps];
+    const size_t n = size_t(std::max<int64_t>(1, g.n_items));
+    T2_HIP(hipMalloc(&g.d_geo, sizeof(t2::ItemGeo<T>) * n));
+    T2_HIP(hipMalloc(reinterpret_cast<void **>(&g.d_topo),
+                     sizeof(t2::ItemTopo) * n));
     std::vector<int64_t> filled(n_steps, 0);
-    std::vector<t2::ProblemDesc<T>> desc(nm);
+    std::vector<t2::ProblemDesc<T>> desc(g.idx.size());
     int64_t *d_off = nullptr;
     T2_HIP(hipMalloc(reinterpret_cast<void **>(&d_off),
                      sizeof(int64_t) * 2 * size_t(n_steps + 1)));
-    for (int i = 0; i < nm; i++) {
-        t2_sweeper *m = b->members[i];
+    for (size_t j = 0; j < g.idx.size(); j++) {
+        t2_sweeper *m = b->members[g.idx[j]];
         const int64_t ms = m->sched.n_steps;
         std::vector<int64_t> dst(ms + 1, 0);
         for (int64_t st = 0; st < ms; st++) {
-            dst[st] = b->step_off[st] + filled[st];
+            dst[st] = g.step_off[st] + filled[st];
             filled[st] += m->sched.step_off[st + 1] - m->sched.step_off[st];
         }
         hipError_t e = hipMemcpy(d_off, m->sched.step_off.data(),
@@ -1342,8 +1421,8 @@ int batch_build(t2_batch *b)
             e = t2::launch_merge_items<T>(
                 static_cast<const t2::ItemGeo<T> *>(m->d_item_geo),
                 m->d_item_topo, d_off, d_off + ms + 1, ms,
-                m->sched.step_off[ms], int32_t(i),
-                static_cast<t2::ItemGeo<T> *>(b->d_geo), b->d_topo, b->stream);
+                m->sched.step_off[ms], int32_t(j),
+                static_cast<t2::ItemGeo<T> *>(g.d_geo), g.d_topo, b->stream);
         if (e == hipSuccess)
             e = hipStreamSynchronize(b->stream);
         if (e != hipSuccess) {
@@ -1351,18 +1430,101 @@ int batch_build(t2_batch *b)
             return fail(T2_ERR_HIP, std::string("batch merge: ") +
                                         hipGetErrorString(e));
         }
-        desc[i].nc = m->nc;
-        desc[i].sigma_mode = m->sigma_mode;
-        desc[i].sigma = static_cast<const T *>(m->d_sigt);
-        desc[i].src_iso = static_cast<const T *>(m->d_src);
-        desc[i].psi = static_cast<T *>(m->d_psi);
-        desc[i].lag_buf = static_cast<const T *>(m->d_lag_buf);
+        desc[j].nc = m->nc;
+        desc[j].sigma_mode = m->sigma_mode;
+        desc[j].sigma = static_cast<const T *>(m->d_sigt);
+        desc[j].src_iso = static_cast<const T *>(m->d_src);
+        desc[j].psi = static_cast<T *>(m->d_psi);
+        desc[j].lag_buf = static_cast<const T *>(m->d_lag_buf);
     }
     (void)hipFree(d_off);
-    b->desc_bytes = sizeof(t2::ProblemDesc<T>) * nm;
-    T2_HIP(hipMalloc(&b->d_desc, b->desc_bytes));
-    T2_HIP(hipMemcpy(b->d_desc, desc.data(), b->desc_bytes,
+    T2_HIP(hipMalloc(&g.d_desc, sizeof(t2::ProblemDesc<T>) * desc.size()));
+    T2_HIP(hipMemcpy(g.d_desc, desc.data(),
+                     sizeof(t2::ProblemDesc<T>) * desc.size(),
                      hipMemcpyHostToDevice));
+    return T2_OK;
+}
+
+int build_levels(t2_batch *b)
+{
+    free_levels(b);
+    std::vector<int> ngs;
+    for (auto *m : b->members)
+        if (std::find(ngs.begin(), ngs.end(), int(m->ng)) == ngs.end())
+            ngs.push_back(int(m->ng));
+    std::sort(ngs.begin(), ngs.end());
+    for (int ng : ngs) {
+        LevelGroup g;
+        g.ng = ng;
+        for (size_t i = 0; i < b->members.size(); i++)
+            if (b->members[i]->ng == ng)
+                g.idx.push_back(int(i));
+        b->groups.push_back(g);
+    }
+    for (auto &g : b->groups)
+        T2_TRY(b->precision == T2_FP64 ? build_level_group<double>(b, g)
+                                       : build_level_group<float>(b, g));
+    b->levels_built = true;
+    return T2_OK;
+}
+
+// Queue form: every member's pending counters and readers table, one
+// problem table, and chunks interleaved step by step over the members.
+template <typename T>
+int build_batch_queue(t2_batch *b)
+{
+    free_batch_queue(b);
+    const int nm = int(b->members.size());
+    std::vector<t2::BatchProblem<T>> probs(nm);
+    for (int i = 0; i < nm; i++) {
+        t2_sweeper *m = b->members[i];
+        if (m->n_chunks < 0)
+            T2_TRY(build_queue(m));
+        t2::BatchProblem<T> &bp = probs[i];
+        bp.a = t2::SweepArgs<T>{};
+        bp.a.geo = static_cast<const t2::ItemGeo<T> *>(m->d_item_geo);
+        bp.a.topo = m->d_item_topo;
+        bp.a.n_items = m->nc * m->na;
+        bp.a.nc = m->nc;
+        bp.a.sigma_mode = m->sigma_mode;
+        bp.a.sigma = static_cast<const T *>(m->d_sigt);
+        bp.a.src_iso = static_cast<const T *>(m->d_src);
+        bp.a.psi = static_cast<T *>(m->d_psi);
+        bp.a.lag_buf = static_cast<const T *>(m->d_lag_buf);
+        bp.pending = m->d_pending;
+        bp.down = m->d_down;
+        bp.groups = int32_t(m->ng);
+    }
+    std::vector<t2::BatchChunk> chunks;
+    for (int64_t st = 0; st < b->n_steps; st++)
+        for (int i = 0; i < nm; i++) {
+            const t2_sweeper *m = b->members[i];
+            if (st >= m->sched.n_steps)
+                continue;
+            const int64_t per = int64_t(t2::kMaxBlock / m->ng) *
+                                b->chunk_rounds;
+            const int64_t lo = m->sched.step_off[st];
+            const int64_t hi = m->sched.step_off[st + 1];
+            for (int64_t c = lo; c < hi; c += per)
+                chunks.push_back({c, int32_t(std::min(per, hi - c)), i});
+        }
+    T2_HIP(hipMalloc(&b->d_problems, sizeof(t2::BatchProblem<T>) * nm));
+    T2_HIP(hipMemcpy(b->d_problems, probs.data(),
+                     sizeof(t2::BatchProblem<T>) * nm, hipMemcpyHostToDevice));
+    T2_HIP(hipMalloc(reinterpret_cast<void **>(&b->d_chunks),
+                     sizeof(t2::BatchChunk) * std::max<size_t>(1, chunks.size())));
+    T2_HIP(hipMemcpy(b->d_chunks, chunks.data(),
+                     sizeof(t2::BatchChunk) * chunks.size(),
+                     hipMemcpyHostToDevice));
+    if (!b->d_ticket) {
+        T2_HIP(hipMalloc(reinterpret_cast<void **>(&b->d_ticket),
+                         sizeof(unsigned long long)));
+        T2_HIP(hipMalloc(reinterpret_cast<void **>(&b->d_abort),
+                         sizeof(int32_t)));
+        T2_HIP(hipMemset(b->d_abort, 0, sizeof(int32_t)));
+    }
+    T2_HIP(t2::queue_batch_grid<T>(&b->grid));
+    b->n_chunks = int64_t(chunks.size());
     return T2_OK;
 }
 
@@ -1377,17 +1539,35 @@ int batch_enqueue(t2_batch *b)
                 m->d_nbr, static_cast<const T *>(m->d_psi),
                 static_cast<T *>(m->d_lag_buf), b->stream));
     }
-    t2::SweepArgs<T> a{};
-    a.problems = static_cast<const t2::ProblemDesc<T> *>(b->d_desc);
-    const auto *geo = static_cast<const t2::ItemGeo<T> *>(b->d_geo);
-    const t2_sweeper *m0 = b->members[0];
-    for (size_t st = 0; st + 1 < b->step_off.size(); st++) {
-        a.geo = geo + b->step_off[st];
-        a.topo = b->d_topo + b->step_off[st];
-        a.n_items = b->step_off[st + 1] - b->step_off[st];
-        T2_HIP(t2::launch_sweep_step<T>(a, int(b->ng), m0->gpl, m0->block,
-                                        b->stream));
+    if (b->engine == T2_ENGINE_QUEUE) {
+        t2::QueueState q{};
+        q.n_chunks = b->n_chunks;
+        q.ticket = b->d_ticket;
+        q.abort = b->d_abort;
+        q.spin_limit = b->spin_limit > 0 ? b->spin_limit : kDefaultSpinLimit;
+        T2_HIP(t2::launch_fill(reinterpret_cast<double *>(b->d_ticket), 0.0,
+                               1, b->stream));
+        T2_HIP(t2::launch_queue_batch<T>(
+            static_cast<const t2::BatchProblem<T> *>(b->d_problems),
+            b->d_chunks, q, b->grid, b->stream));
+        return T2_OK;
     }
+    int64_t longest = 0;
+    for (auto &g : b->groups)
+        longest = std::max<int64_t>(longest, int64_t(g.step_off.size()) - 1);
+    for (int64_t st = 0; st < longest; st++)
+        for (auto &g : b->groups) {
+            if (st + 1 >= int64_t(g.step_off.size()))
+                continue;
+            t2::SweepArgs<T> a{};
+            a.problems = static_cast<const t2::ProblemDesc<T> *>(g.d_desc);
+            a.geo = static_cast<const t2::ItemGeo<T> *>(g.d_geo) +
+                    g.step_off[st];
+            a.topo = g.d_topo + g.step_off[st];
+            a.n_items = g.step_off[st + 1] - g.step_off[st];
+            T2_HIP(t2::launch_sweep_step<T>(a, g.ng, 1, t2::kMaxBlock,
+                                            b->stream));
+        }
     return T2_OK;
 }
 
@@ -1412,6 +1592,21 @@ int members_after_batch(t2_batch *b)
 
 int batch_sweep(t2_batch *b)
 {
+    if (b->engine == T2_ENGINE_QUEUE && b->n_chunks < 0) {
+        drop_batch_graph(b);
+        T2_HIP(hipStreamSynchronize(b->stream));
+        T2_TRY(b->precision == T2_FP64 ? build_batch_queue<double>(b)
+                                       : build_batch_queue<float>(b));
+        batch_snapshot(b);
+    }
+    if (b->engine == T2_ENGINE_LEVELS && !b->levels_built) {
+        drop_batch_graph(b);
+        T2_HIP(hipStreamSynchronize(b->stream));
+        T2_TRY(build_levels(b));
+        batch_snapshot(b);
+    }
+    for (auto *m : b->members)
+        m->n_sweeps++;
     if (!b->graph) {
         hipGraph_t g = nullptr;
         T2_HIP(hipStreamBeginCapture(b->stream,
@@ -1433,6 +1628,28 @@ int batch_sweep(t2_batch *b)
     return T2_OK;
 }
 
+// After an abandoned batched ready-queue sweep: reports it once and re-arms
+// every member's counters.
+int batch_check_abort(t2_batch *b)
+{
+    if (!b->d_abort)
+        return T2_OK;
+    int32_t flag = 0;
+    T2_HIP(hipStreamSynchronize(b->stream));
+    T2_HIP(hipMemcpy(&flag, b->d_abort, sizeof(flag), hipMemcpyDeviceToHost));
+    if (!flag)
+        return T2_OK;
+    for (auto *m : b->members)
+        if (m->d_pending)
+            T2_HIP(t2::launch_queue_rearm(m->d_item_topo, m->nc * m->na,
+                                          m->nc, m->d_pending, b->stream));
+    T2_HIP(hipMemsetAsync(b->d_abort, 0, sizeof(int32_t), b->stream));
+    T2_HIP(hipStreamSynchronize(b->stream));
+    return fail(T2_ERR_TIMEOUT,
+                "a ready-queue wait ran past its poll limit; the batched "
+                "sweep was abandoned and its results are invalid");
+}
+
 }  // namespace
 
 extern "C" {
@@ -1447,10 +1664,10 @@ int t2_batch_create(t2_batch **out, t2_sweeper *const *members, int count)
         const t2_sweeper *m = members[i];
         if (!m)
             return fail(T2_ERR_ARG, "t2_batch_create: null member");
-        if (m->device != members[0]->device || m->ng != members[0]->ng ||
+        if (m->device != members[0]->device ||
             m->precision != members[0]->precision)
-            return fail(T2_ERR_ARG, "batch members need the same device, "
-                                    "group count and precision");
+            return fail(T2_ERR_ARG, "batch members need the same device and "
+                                    "precision");
         if (m->d_qang)
             return fail(T2_ERR_ARG, "batch members need isotropic sources");
         for (int j = 0; j < i; j++)
@@ -1463,7 +1680,10 @@ int t2_batch_create(t2_batch **out, t2_sweeper *const *members, int count)
     b->members.assign(members, members + count);
     b->device = members[0]->device;
     b->precision = members[0]->precision;
-    b->ng = members[0]->ng;
+    for (auto *m : b->members) {
+        b->n_steps = std::max(b->n_steps, m->sched.n_steps);
+        b->n_items += m->nc * m->na;
+    }
     if (hipSetDevice(b->device) != hipSuccess ||
         hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) !=
             hipSuccess) {
@@ -1476,12 +1696,7 @@ int t2_batch_create(t2_batch **out, t2_sweeper *const *members, int count)
     for (auto *m : b->members)
         if (hipStreamSynchronize(m->stream) != hipSuccess)
             return batch_fail_cleanup(b, fail(T2_ERR_HIP, "member sync"));
-    int rc = b->precision == T2_FP64 ? batch_build<double>(b)
-                                     : batch_build<float>(b);
-    if (rc != T2_OK)
-        return batch_fail_cleanup(b, rc);
-    for (auto *m : b->members)
-        b->snap.push_back(member_buffers(m));
+    batch_snapshot(b);
     *out = b;
     return T2_OK;
 }
@@ -1493,19 +1708,45 @@ int t2_batch_destroy(t2_batch *b)
     (void)hipSetDevice(b->device);
     if (b->stream)
         (void)hipStreamSynchronize(b->stream);
-    if (b->graph)
-        (void)hipGraphExecDestroy(b->graph);
-    if (b->d_geo)
-        (void)hipFree(b->d_geo);
-    if (b->d_topo)
-        (void)hipFree(b->d_topo);
-    if (b->d_desc)
-        (void)hipFree(b->d_desc);
+    drop_batch_graph(b);
+    free_levels(b);
+    free_batch_queue(b);
+    if (b->d_ticket)
+        (void)hipFree(b->d_ticket);
+    if (b->d_abort)
+        (void)hipFree(b->d_abort);
     if (b->order)
         (void)hipEventDestroy(b->order);
     if (b->stream)
         (void)hipStreamDestroy(b->stream);
     delete b;
+    return T2_OK;
+}
+
+int t2_batch_set_engine(t2_batch *b, int engine, int chunk_rounds,
+                        int64_t spin_limit)
+{
+    t2::clear_error();
+    if (!b)
+        return fail(T2_ERR_ARG, "null batch");
+    if (engine != T2_ENGINE_LEVELS && engine != T2_ENGINE_QUEUE)
+        return fail(T2_ERR_ARG, "engine must be T2_ENGINE_LEVELS or "
+                                "T2_ENGINE_QUEUE");
+    if (spin_limit < 0)
+        return fail(T2_ERR_ARG, "spin_limit < 0");
+    if (chunk_rounds < 0 || chunk_rounds > 256)
+        return fail(T2_ERR_ARG, "chunk_rounds must be in [0, 256]");
+    const int rounds = chunk_rounds ? chunk_rounds : 4;
+    T2_HIP(hipSetDevice(b->device));
+    T2_HIP(hipStreamSynchronize(b->stream));
+    if (engine != b->engine || spin_limit != b->spin_limit ||
+        rounds != b->chunk_rounds)
+        drop_batch_graph(b);
+    if (rounds != b->chunk_rounds)
+        free_batch_queue(b);
+    b->engine = engine;
+    b->chunk_rounds = rounds;
+    b->spin_limit = spin_limit;
     return T2_OK;
 }
 
@@ -1556,7 +1797,7 @@ int t2_batch_info(const t2_batch *b, int64_t *n_steps, int64_t *n_items)
     if (!b)
         return fail(T2_ERR_ARG, "null batch");
     if (n_steps)
-        *n_steps = int64_t(b->step_off.size()) - 1;
+        *n_steps = b->n_steps;
     if (n_items)
         *n_items = b->n_items;
     return T2_OK;
@@ -1569,7 +1810,7 @@ int t2_batch_synchronize(t2_batch *b)
         return fail(T2_ERR_ARG, "null batch");
     T2_HIP(hipSetDevice(b->device));
     T2_HIP(hipStreamSynchronize(b->stream));
-    return T2_OK;
+    return batch_check_abort(b);
 }
 
 int t2_batch_get_stream(const t2_batch *b, void **stream)

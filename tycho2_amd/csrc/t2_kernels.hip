@@ -15,6 +15,9 @@ namespace t2 {
 namespace {
 
 constexpr double kInv12 = 1.0 / 12.0;
+// s_waitcnt operand (gfx9 encoding) waiting for vmcnt = 0 only: expcnt and
+// lgkmcnt fields at their maxima.
+constexpr int kWaitVmcnt0 = 0x0F70;
 constexpr double kInv20 = 1.0 / 20.0;
 
 // Omega . n with each product and sum rounded separately: the host level
@@ -153,8 +156,11 @@ __global__ void build_items_kernel(const uint32_t *__restrict__ work,
 
 // ---------------------------------------------------------------- sweep
 // Cell solve of work item `item` (topology record t) for the groups of lane
-// gl: gl, gl + LPI, ... (LPI = G / GPL lanes per item).
-template <typename T, int G, int GPL>
+// gl: gl, gl + LPI, ... (LPI = G / GPL lanes per item).  COH: upwind psi is
+// read and psi written at the device coherence point (sc1 loads, write-
+// through stores), as the ready-queue kernel needs; otherwise plain cached
+// accesses (kernel boundaries order the per-step launches).
+template <typename T, int G, int GPL, bool COH = false>
 __device__ __forceinline__ void solve_item(const SweepArgs<T> &p,
                                            int64_t item, int gl,
                                            const ItemTopo &t)
@@ -243,7 +249,9 @@ __device__ __forceinline__ void solve_item(const SweepArgs<T> &p,
                         continue;
                     }
                     const uint32_t vtx = (t.meta >> (6 * k + 2 * slot)) & 3u;
-                    uv[j] = pn[vtx * G];
+                    uv[j] = COH ? __hip_atomic_load(pn + vtx * G, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT)
+                                : pn[vtx * G];
                     slot++;
                 }
             } else {
@@ -272,8 +280,13 @@ __device__ __forceinline__ void solve_item(const SweepArgs<T> &p,
 
         T *dst = psi + size_t(t.angle) * plane + own + g;
 #pragma unroll
-        for (int v = 0; v < 4; v++)
-            dst[v * G] = b[v];
+        for (int v = 0; v < 4; v++) {
+            if constexpr (COH)
+                __hip_atomic_store(dst + v * G, b[v], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            else
+                dst[v * G] = b[v];
+        }
     }
 }
 
@@ -293,82 +306,150 @@ sweep_step_kernel(const SweepArgs<T> p)
 
 // ---------------------------------------------------------------- ready queue
 // The whole sweep in one persistent launch.  Blocks take chunks of work items
-// from a global ticket counter in step order (a chunk never spans two
-// steps); an item waits until its pending-upwind counter reaches zero, solves,
-// then decrements the counters of the items that read its psi.  Every
-// dependency of a chunk lies in an earlier chunk, already held by a running
-// block whose own dependencies are earlier still, so the lowest outstanding
-// ticket can always proceed: no assumption on how many blocks are resident.
-// Each wait is bounded by spin_limit polls; a wait that runs out sets the
-// abort flag, after which every block stops taking work and exits.
+// from a global ticket counter in step order (a chunk never spans two steps)
+// and split each chunk between their wavefronts.  A wavefront first waits
+// until the pending-upwind counter of each of its items reaches zero, then
+// solves them all, then decrements the counters of the items that read their
+// psi.  Every dependency of a chunk lies in an earlier chunk, already held by
+// a running block whose own dependencies are earlier still, so the lowest
+// outstanding ticket can always proceed: nothing assumes how many blocks are
+// resident.  Each wait is bounded by spin_limit polls; a wait that runs out
+// sets the abort flag, after which every block stops taking work and exits.
+//
+// Cross-XCD visibility: psi is written through to the coherence point and
+// upwind psi read from it (sc1 stores and loads), the counters are device-
+// scope atomics, and the stores are complete (vmcnt drained) before any
+// decrement is issued.  No L2-wide write-back or invalidate is needed, which
+// agent-scope release/acquire fences would issue per wavefront and chunk.
+// One wavefront's share of a chunk [lo, hi) of one problem's items: wait,
+// solve, release.
 template <typename T, int G, int GPL>
-__global__ void __launch_bounds__(kMaxBlock)
-queue_sweep_kernel(const SweepArgs<T> p, const QueueState q)
+__device__ __forceinline__ void queue_chunk(const SweepArgs<T> &p,
+                                            int32_t *pending,
+                                            const int4 *down, int64_t lo,
+                                            int64_t hi, const QueueState &q)
 {
-    constexpr int LPI = G / GPL;
-    __shared__ int64_t s_chunk;
-    const int64_t slot = threadIdx.x / LPI;
-    const int gl = int(threadIdx.x % LPI);
-    for (;;) {
-        if (threadIdx.x == 0) {
-            int64_t c = int64_t(__hip_atomic_fetch_add(
-                q.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            if (__hip_atomic_load(q.abort, __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT) != 0)
-                c = q.n_chunks;
-            s_chunk = c;
-        }
-        __syncthreads();
-        const int64_t c = s_chunk;
-        __syncthreads();
-        if (c >= q.n_chunks)
-            break;
-        const int64_t item = q.chunk_off[c] + slot;
-        const bool active = item < q.chunk_off[c + 1];
-        ItemTopo t{};
-        int32_t *mine = nullptr;
-        bool ok = active;
-        if (active) {
-            t = p.topo[item];
-            mine = q.pending + size_t(t.angle) * size_t(p.nc) + t.cell;
-            int64_t spins = 0;
-            while (__hip_atomic_load(mine, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT) > 0) {
-                if (++spins > q.spin_limit) {
-                    __hip_atomic_store(q.abort, 1, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    ok = false;
-                    break;
-                }
-                if ((spins & 63) == 0 &&
-                    __hip_atomic_load(q.abort, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT) != 0) {
-                    ok = false;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
+    constexpr int LPI = G / GPL;          // lanes per item
+    constexpr int IPR = 64 / LPI;         // items per wavefront round
+    const int wave = int(threadIdx.x / 64), lane = int(threadIdx.x % 64);
+    const int gl = lane % LPI;
+    const int n_waves = int(blockDim.x / 64);
+    // this wavefront's share of the chunk, a whole number of rounds
+    const int64_t span =
+        ((hi - lo + n_waves - 1) / n_waves + IPR - 1) / IPR * IPR;
+    const int64_t w_lo = lo + wave * span;
+    const int64_t w_hi = w_lo + span < hi ? w_lo + span : hi;
+    const int64_t first = w_lo + lane / LPI;
+    bool ok = true;
+    for (int64_t it = first; ok && it < w_hi; it += IPR) {
+        const ItemTopo t = p.topo[it];
+        const int32_t *mine = pending + size_t(t.angle) * size_t(p.nc) + t.cell;
+        int64_t spins = 0;
+        while (__hip_atomic_load(mine, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT) > 0) {
+            if (++spins > q.spin_limit) {
+                __hip_atomic_store(q.abort, 1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                ok = false;
+                break;
             }
+            if ((spins & 63) == 0 &&
+                __hip_atomic_load(q.abort, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                ok = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
         }
-        // upwind psi written before the counters reached zero is visible
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        if (ok)
-            solve_item<T, G, GPL>(p, item, gl, t);
-        // this wavefront's psi stores are visible before any decrement
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (ok && gl == 0) {
+    }
+    // no upwind load may move above the waits (compiler ordering; the loads
+    // themselves read the coherence point)
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    for (int64_t it = first; ok && it < w_hi; it += IPR) {
+        const ItemTopo t = p.topo[it];
+        solve_item<T, G, GPL, true>(p, it, gl, t);
+    }
+    // every psi store of this wavefront is complete before a decrement
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    if (ok && gl == 0) {
+        for (int64_t it = first; it < w_hi; it += IPR) {
+            const ItemTopo t = p.topo[it];
+            int32_t *row = pending + size_t(t.angle) * size_t(p.nc);
             const int n_up = (t.up[0] >= 0) + (t.up[1] >= 0) +
                              (t.up[2] >= 0) + (t.up[3] >= 0);
             // re-armed for the next sweep (nothing else touches it now)
-            __hip_atomic_store(mine, n_up, __ATOMIC_RELAXED,
+            __hip_atomic_store(row + t.cell, n_up, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
-            const int4 d = q.down[item];
+            const int4 d = down[it];
             const int32_t dn[4] = {d.x, d.y, d.z, d.w};
-            int32_t *row = q.pending + size_t(t.angle) * size_t(p.nc);
 #pragma unroll
             for (int k = 0; k < 4; k++)
                 if (dn[k] >= 0)
                     __hip_atomic_fetch_add(row + dn[k], -1, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Next chunk for the whole block, or a value >= n_chunks once the work is
+// handed out or the sweep was abandoned.
+__device__ __forceinline__ int64_t next_chunk(const QueueState &q,
+                                              int64_t *s_chunk)
+{
+    if (threadIdx.x == 0) {
+        int64_t c = int64_t(__hip_atomic_fetch_add(
+            q.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (__hip_atomic_load(q.abort, __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT) != 0)
+            c = q.n_chunks;
+        *s_chunk = c;
+    }
+    __syncthreads();
+    const int64_t c = *s_chunk;
+    __syncthreads();
+    return c;
+}
+
+template <typename T, int G, int GPL>
+__global__ void __launch_bounds__(kMaxBlock)
+queue_sweep_kernel(const SweepArgs<T> p, const QueueState q)
+{
+    __shared__ int64_t s_chunk;
+    for (;;) {
+        const int64_t c = next_chunk(q, &s_chunk);
+        if (c >= q.n_chunks)
+            break;
+        queue_chunk<T, G, GPL>(p, q.pending, q.down, q.chunk_off[c],
+                               q.chunk_off[c + 1], q);
+    }
+}
+
+// Several independent problems in one persistent launch: the chunk table
+// interleaves the problems step by step, each chunk names its problem, and
+// the problem's group count selects the lane mapping.
+template <typename T>
+__global__ void __launch_bounds__(kMaxBlock)
+queue_batch_kernel(const BatchProblem<T> *problems, const BatchChunk *chunks,
+                   const QueueState q)
+{
+    __shared__ int64_t s_chunk;
+    for (;;) {
+        const int64_t c = next_chunk(q, &s_chunk);
+        if (c >= q.n_chunks)
+            break;
+        const BatchChunk ch = chunks[c];
+        const BatchProblem<T> &bp = problems[ch.problem];
+        const int64_t lo = ch.lo, hi = ch.lo + ch.n;
+        switch (bp.groups) {
+        case 1: queue_chunk<T, 1, 1>(bp.a, bp.pending, bp.down, lo, hi, q); break;
+        case 2: queue_chunk<T, 2, 1>(bp.a, bp.pending, bp.down, lo, hi, q); break;
+        case 4: queue_chunk<T, 4, 1>(bp.a, bp.pending, bp.down, lo, hi, q); break;
+        case 8: queue_chunk<T, 8, 1>(bp.a, bp.pending, bp.down, lo, hi, q); break;
+        case 16: queue_chunk<T, 16, 1>(bp.a, bp.pending, bp.down, lo, hi, q); break;
+        case 32: queue_chunk<T, 32, 1>(bp.a, bp.pending, bp.down, lo, hi, q); break;
+        case 64: queue_chunk<T, 64, 1>(bp.a, bp.pending, bp.down, lo, hi, q); break;
+        default: break;
         }
     }
 }
@@ -927,6 +1008,35 @@ hipError_t queue_grid(int groups, int groups_per_lane, int block,
                         nullptr, grid);
 }
 
+template <typename T>
+hipError_t launch_queue_batch(const BatchProblem<T> *problems,
+                              const BatchChunk *chunks, const QueueState &q,
+                              int64_t grid, hipStream_t stream)
+{
+    if (q.n_chunks <= 0)
+        return hipSuccess;
+    grid = std::min<int64_t>(std::max<int64_t>(1, grid), q.n_chunks);
+    hipLaunchKernelGGL((queue_batch_kernel<T>), dim3(unsigned(grid)),
+                       dim3(kMaxBlock), 0, stream, problems, chunks, q);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t queue_batch_grid(int64_t *grid)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                  dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(&queue_batch_kernel<T>),
+            kMaxBlock, 0);
+    *grid = int64_t(std::max(1, per_cu)) * std::max(1, cus);
+    return e;
+}
+
 hipError_t launch_queue_links(const ItemTopo *topo, int64_t n_items,
                               int64_t nc, int32_t *pos, int32_t *dcount,
                               int4 *down, int32_t *pending, int *err,
@@ -1126,6 +1236,11 @@ hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t stream)
                                               const QueueState &, int, int,   \
                                               int, int64_t, hipStream_t);     \
     template hipError_t queue_grid<T>(int, int, int, int64_t *);              \
+    template hipError_t launch_queue_batch<T>(const BatchProblem<T> *,        \
+                                              const BatchChunk *,             \
+                                              const QueueState &, int64_t,    \
+                                              hipStream_t);                   \
+    template hipError_t queue_batch_grid<T>(int64_t *);                       \
     template hipError_t launch_merge_items<T>(                                \
         const ItemGeo<T> *, const ItemTopo *, const int64_t *,                \
         const int64_t *, int64_t, int64_t, int32_t, ItemGeo<T> *,             \

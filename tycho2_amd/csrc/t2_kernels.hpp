@@ -148,6 +148,26 @@ template <typename T>
 hipError_t queue_grid(int groups, int groups_per_lane, int block,
                       int64_t *grid);
 
+// Batched ready queue: per-problem tables and the interleaved chunk list.
+template <typename T>
+struct BatchProblem {
+    SweepArgs<T> a;                // records (whole sweep), arrays, nc
+    int32_t *pending;              // the problem's [na][nc] counters
+    const int4 *down;              // the problem's readers table
+    int32_t groups;                // 1, 2, 4, ..., 64
+};
+struct BatchChunk {
+    int64_t lo;                    // first item, in the problem's step order
+    int32_t n;                     // items
+    int32_t problem;
+};
+template <typename T>
+hipError_t launch_queue_batch(const BatchProblem<T> *problems,
+                              const BatchChunk *chunks, const QueueState &q,
+                              int64_t grid, hipStream_t stream);
+template <typename T>
+hipError_t queue_batch_grid(int64_t *grid);
+
 // Builds down[] and pending[] from the item records (pos and dcount: scratch
 // of na * nc and n_items int32); *err is set if an item had > 4 readers.
 hipError_t launch_queue_links(const ItemTopo *topo, int64_t n_items,

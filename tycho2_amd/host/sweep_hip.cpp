@@ -7,7 +7,9 @@
 // through the engine's C ABI (include/tycho2_sweep.h); the host-side source
 // iteration below calls it exactly as the reference's
 // SourceIteration::fixedPoint does (src/SourceIteration.cc:127-207), and
-// "DeviceLoop true" in the deck runs the outer iteration on the device.
+// "DeviceLoop true" in the deck runs the outer iteration on the device;
+// "Precision FP32" and "SweepEngine Queue" select the engine's FP32 sweeps
+// and its persistent ready-queue launch.
 //
 // Usage: sweep_hip.x <mesh.smesh> <input.deck>
 #include <cmath>
@@ -273,12 +275,21 @@ public:
     {
         const int prec = deck.str("Precision", "FP64") == "FP32" ? T2_FP32
                                                                  : T2_FP64;
+        // "SweepEngine Queue" selects the persistent ready-queue launch
+        // (default: per-step launches replayed from a hipGraph)
+        const std::string engine = deck.str("SweepEngine", "Levels");
+        if (engine != "Levels" && engine != "Queue")
+            throw std::runtime_error("SweepEngine must be Levels or Queue");
         check(t2_sweeper_create(&c_engine, 0, p.mesh.nc, p.mesh.nn,
                                 p.mesh.coords.data(), p.mesh.cell_nodes.data(),
                                 p.mesh.adj.data(), p.mesh.f2c.data(), p.na,
                                 p.omega.data(), p.wt.data(), p.ng,
                                 p.sigma_t.data(), p.sigma_s.data(), prec,
                                 T2_GAUSS_NOPIVOT));
+        check(t2_set_engine(c_engine,
+                            engine == "Queue" ? T2_ENGINE_QUEUE
+                                              : T2_ENGINE_LEVELS,
+                            0, 0));
     }
     ~SweeperHIP() override { t2_sweeper_destroy(c_engine); }
 

@@ -48,7 +48,7 @@ SIGNATURES = {
                                      ctypes.c_int]),
     "t2_set_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "t2_set_engine": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
-                                     ctypes.c_int64]),
+                                     ctypes.c_int, ctypes.c_int64]),
     "t2_solve_refined": (ctypes.c_int, [
         ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
         ctypes.c_int64, ctypes.c_void_p, _dp]),
@@ -75,6 +75,8 @@ SIGNATURES = {
                                        ctypes.POINTER(ctypes.c_void_p),
                                        ctypes.c_int]),
     "t2_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "t2_batch_set_engine": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int64]),
     "t2_batch_sweep_step": (ctypes.c_int, [ctypes.c_void_p]),
     "t2_batch_sweep_only": (ctypes.c_int, [ctypes.c_void_p]),
     "t2_batch_info": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip]),
@@ -90,7 +92,13 @@ class T2Info(ctypes.Structure):
                 ("min_levels", ctypes.c_int64), ("n_lagged", ctypes.c_int64),
                 ("psi_bytes", ctypes.c_int64),
                 ("device_bytes", ctypes.c_int64),
-                ("precision", ctypes.c_int32), ("gauss", ctypes.c_int32)]
+                ("precision", ctypes.c_int32), ("gauss", ctypes.c_int32),
+                ("sweeps", ctypes.c_int64), ("engine", ctypes.c_int32),
+                ("angle_batches", ctypes.c_int32),
+                ("items_per_block", ctypes.c_int32),
+                ("groups_per_lane", ctypes.c_int32),
+                ("chunk_rounds", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class T2RefineInfo(ctypes.Structure):
@@ -235,11 +243,14 @@ class SweeperHIP:
     def set_schedule(self, angle_batches: int) -> None:
         _check(lib().t2_set_schedule(self.handle, int(angle_batches)))
 
-    def set_engine(self, engine: int, spin_limit: int = 0) -> None:
+    def set_engine(self, engine: int, spin_limit: int = 0,
+                   chunk_rounds: int = 0) -> None:
         """ENGINE_LEVELS (per-step launches in a hipGraph) or ENGINE_QUEUE
         (one persistent ready-queue launch); spin_limit bounds each queue
-        wait in polls (0: library default)."""
-        _check(lib().t2_set_engine(self.handle, int(engine), int(spin_limit)))
+        wait in polls, chunk_rounds sets the queue's chunk size (0: library
+        defaults)."""
+        _check(lib().t2_set_engine(self.handle, int(engine), int(chunk_rounds),
+                                   int(spin_limit)))
 
     # -- SweeperAbstract::sweep
     def sweep(self, psi: np.ndarray, source: np.ndarray,
@@ -330,12 +341,14 @@ class SweeperHIP:
 
 
 class SweepBatch:
-    """Several sweepers swept together, one launch per merged step.
+    """Several sweepers (any meshes, quadratures and group counts; one
+    precision) swept together: merged per-step launches (ENGINE_LEVELS) or
+    one persistent ready-queue launch for all of them (ENGINE_QUEUE).
 
     Members keep their own psi, phi and sources; each batch sweep gives every
     member exactly what its own sweep would.  Members must stay alive and
-    keep their schedule, launch shape and source kind while the batch is
-    used (the library refuses a batch whose members changed)."""
+    keep their schedule and source kind while the batch is used (the
+    library refuses a batch whose members changed)."""
 
     def __init__(self, members):
         self.members = list(members)
@@ -367,6 +380,11 @@ class SweepBatch:
         if not self._h:
             raise T2Error(-4, "batch is closed")
         return self._h
+
+    def set_engine(self, engine: int, spin_limit: int = 0,
+                   chunk_rounds: int = 0) -> None:
+        _check(lib().t2_batch_set_engine(self.handle, int(engine),
+                                         int(chunk_rounds), int(spin_limit)))
 
     def sweep_step(self) -> None:
         _check(lib().t2_batch_sweep_step(self.handle))
