@@ -67,6 +67,8 @@ def path_leg(p, precision, steps, warmup, ev, tune, solve):
             out["tuned"] = best
             log(f"autotune: {best}")
         info = s.info()
+        out["engine"] = info["engine"]
+        out["queue_first_step"] = info["queue_first_step"]
         for _ in range(warmup):
             s.sweep_step()
         s.synchronize()
@@ -211,9 +213,13 @@ def main() -> int:
         t = r.get("tuned") or {}
         gpl = t.get("groups_per_lane", 1)
         ty = "double" if width == 8 else "float"
-        if t.get("engine", 0) == 1:
+        if r.get("engine", 0) == 1:
             kernel = f"queue_sweep_kernel<{ty},{p.ng},{gpl}>"
             launches = 1
+        elif r.get("engine", 0) == 2:
+            kernel = (f"sweep_step_kernel<{ty},{p.ng},{gpl}> (steps before "
+                      f"{r['queue_first_step']}) + queue_sweep_kernel")
+            launches = r["queue_first_step"] + 1
         else:
             kernel = f"sweep_step_kernel<{ty},{p.ng},{gpl}>"
             launches = r["n_steps"]

@@ -72,7 +72,7 @@ typedef struct {
     int32_t items_per_block;  /* launch knobs (t2_set_launch)              */
     int32_t groups_per_lane;
     int32_t chunk_rounds;     /* queue knob (t2_set_engine)                */
-    int32_t reserved;
+    int32_t queue_first_step; /* first step run by the queue (-1: none)    */
 } t2_info;
 
 /* Message of the last failure on this thread ("" if none). */
@@ -127,13 +127,25 @@ int t2_set_launch(t2_sweeper *s, int items_per_block, int groups_per_lane);
  * do not depend on it. */
 int t2_set_schedule(t2_sweeper *s, int angle_batches);
 
+/* Order of the items within a step (any order is valid; results do not
+ * depend on it): T2_ORDER_CELL by angle then cell (default); T2_ORDER_FACES
+ * by angle, then the cell's pattern of incoming interior faces, then cell,
+ * so that the items sharing a wavefront read the same faces. */
+#define T2_ORDER_CELL 0
+#define T2_ORDER_FACES 1
+int t2_set_order(t2_sweeper *s, int order);
+
 /* Sweep engine (GraphTraverser::traverse, src/GraphTraverser.cc:467-759):
  *   T2_ENGINE_LEVELS  one kernel launch per schedule step, the whole sweep
  *                     captured once in a hipGraph and replayed (default);
  *   T2_ENGINE_QUEUE   one persistent launch: blocks take chunks of items in
  *                     schedule order from a device ticket counter, each item
  *                     waits on its atomic pending-upwind counter, solves, and
- *                     decrements the counters of the cells that read it.
+ *                     decrements the counters of the cells that read it;
+ *   T2_ENGINE_HYBRID  per-step launches while a step fills the GPU, then one
+ *                     ready-queue launch for the thin tail of the schedule
+ *                     (the steps that hold fewer items than the queue kernel
+ *                     keeps resident; t2_info.queue_first_step).
  * Results are bitwise the same.  chunk_rounds: items per lane slot in one
  * queue chunk (a chunk is chunk_rounds x items_per_block items; 0 = 4).
  * spin_limit bounds every queue wait (polls; 0 = default); a wait that runs
@@ -142,8 +154,12 @@ int t2_set_schedule(t2_sweeper *s, int angle_batches);
  * valid). */
 #define T2_ENGINE_LEVELS 0
 #define T2_ENGINE_QUEUE 1
+#define T2_ENGINE_HYBRID 2
 int t2_set_engine(t2_sweeper *s, int engine, int chunk_rounds,
                   int64_t spin_limit);
+/* Hybrid engine: first step run by the ready queue (steps before it are
+ * launched one by one); -1 (default) picks the thin tail automatically. */
+int t2_set_queue_split(t2_sweeper *s, int64_t first_step);
 
 /* ------------------------------------------------------------ drop-in sweep
  * SweeperAbstract::sweep(psi, source, zeroPsiBound) on host arrays in the

@@ -11,8 +11,8 @@ import pytest
 import oracle as orc
 from tycho2_amd import mesh as meshmod
 from tycho2_amd.quadrature import product_quadrature
-from tycho2_amd.sweeper import (ENGINE_LEVELS, ENGINE_QUEUE, ERR_TIMEOUT,
-                                FP32, FP64, SweeperHIP, T2Error)
+from tycho2_amd.sweeper import (ENGINE_HYBRID, ENGINE_LEVELS, ENGINE_QUEUE,
+                                ERR_TIMEOUT, FP32, FP64, SweeperHIP, T2Error)
 
 pytestmark = pytest.mark.gpu
 
@@ -31,11 +31,14 @@ def problem(n, sn, ng, seed, jitter=0.15):
 
 
 def run_steps(args, precision, engine, steps=3, batches=1, launch=None,
-              rounds=0):
+              rounds=0, split=None):
     sm, conn, omega, wt, st, ss, q = args
     ng = q.shape[2]
     with SweeperHIP(sm, conn, omega, wt, ng, st, ss, precision) as s:
         s.set_engine(engine, SPIN, rounds)
+        if split is not None:
+            n = s.info()["n_steps"]
+            s.set_queue_split({"half": n // 2, "all": n}.get(split, split))
         if batches != 1:
             s.set_schedule(batches)
         if launch:
@@ -72,9 +75,23 @@ def test_queue_schedules_and_shapes_bitwise(batches, launch, rounds):
     assert np.array_equal(ref[1], got[1])
 
 
-def test_queue_lagged_sweep_matches_oracle():
+@pytest.mark.parametrize("precision,ng,split", [
+    (FP64, 16, "half"), (FP32, 16, "half"), (FP64, 1, 0), (FP64, 4, "all"),
+    (FP64, 4, -1), (FP32, 1, 3)])
+def test_hybrid_matches_levels_bitwise(precision, ng, split):
+    """Per-step launches up to the split step, the ready queue after it."""
+    print(f"hybrid precision={precision} ng={ng} split={split}", flush=True)
+    args = problem(5, 4, ng, seed=80 + ng)
+    ref = run_steps(args, precision, ENGINE_LEVELS)
+    got = run_steps(args, precision, ENGINE_HYBRID, split=split)
+    assert np.array_equal(ref[0], got[0])
+    assert np.array_equal(ref[1], got[1])
+
+
+@pytest.mark.parametrize("engine", [ENGINE_QUEUE, ENGINE_HYBRID])
+def test_queue_lagged_sweep_matches_oracle(engine):
     """Cyclic mesh: lagged faces read the pre-sweep psi in the queue too."""
-    print("queue cyclic sweep", flush=True)
+    print(f"queue cyclic sweep engine={engine}", flush=True)
     sm = meshmod.kuhn_cube(3, 0.2, seed=3)
     conn = meshmod.tycho_connectivity(sm)
     omega, wt = orc.quadrature(2)
@@ -84,9 +101,12 @@ def test_queue_lagged_sweep_matches_oracle():
     prev = rng.uniform(0.0, 1.0, src.shape)
     psi = prev.copy()
     with SweeperHIP(sm, conn, omega, wt, 1, st, 0.5 * st, FP64) as s:
-        s.set_engine(ENGINE_QUEUE, SPIN)
+        s.set_engine(engine, SPIN)
+        s.set_queue_split(s.info()["n_steps"] // 2)
         assert s.info()["n_lagged"] > 0
         s.sweep(psi, src)
+        if engine == ENGINE_HYBRID:
+            assert s.info()["queue_first_step"] == s.info()["n_steps"] // 2
     ref, _ = orc.sweep(orc.build_mesh(sm), omega, 1, st, src, psi=prev.copy())
     assert np.abs(psi - ref).max() <= 1e-12 * np.abs(ref).max()
 
@@ -128,3 +148,23 @@ def test_queue_wait_limit_abandons_and_recovers():
         s.set_phi(np.zeros_like(ref[0]))
         s.sweep_step()
         assert np.array_equal(s.phi(), ref[0])
+
+
+@pytest.mark.parametrize("engine", [ENGINE_LEVELS, ENGINE_QUEUE])
+def test_face_order_bitwise(engine):
+    """Items sorted by incoming-face pattern within a step: same results."""
+    from tycho2_amd.sweeper import ORDER_FACES
+    print(f"face order engine={engine}", flush=True)
+    sm, conn, omega, wt, st, ss, q = problem(5, 4, 16, seed=90)
+    out = []
+    for order in (None, ORDER_FACES):
+        with SweeperHIP(sm, conn, omega, wt, 16, st, ss, FP64) as s:
+            s.set_engine(engine, SPIN)
+            if order is not None:
+                s.set_order(order)
+            s.set_source_isotropic(q)
+            for _ in range(2):
+                s.sweep_step()
+            out.append((s.phi(), s.psi_angle(5)))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])

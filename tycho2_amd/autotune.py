@@ -6,14 +6,15 @@ stream) over
     after the other (smaller per-step working set, more steps),
   * cells per block and groups per lane -- the launch shape of the sweep
     kernels,
-  * the engine -- per-step launches replayed from a hipGraph, or one
-    persistent ready-queue launch,
+  * the engine -- per-step launches replayed from a hipGraph, one
+    persistent ready-queue launch, or per-step launches with the schedule's
+    thin tail on the ready queue (and the queue's chunk size),
 keeps the fastest setting on the handle and returns the measurements.
 Results do not depend on any of these knobs (bitwise, tested).
 """
 from __future__ import annotations
 
-from .sweeper import ENGINE_LEVELS, ENGINE_QUEUE
+from .sweeper import ENGINE_HYBRID, ENGINE_LEVELS, ENGINE_QUEUE
 from .timing import HipEvents, time_region
 
 BATCHES = (1, 2, 4, 8, 16)
@@ -30,7 +31,7 @@ def _time_sweep(ev, s, reps):
 
 def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
          groups_per_lane=GROUPS_PER_LANE,
-         engines=(ENGINE_LEVELS, ENGINE_QUEUE), log=None):
+         engines=(ENGINE_LEVELS, ENGINE_QUEUE, ENGINE_HYBRID), log=None):
     """Tune handle `s` in place; returns (best, trials) where best is a dict
     {engine, batches, block, groups_per_lane, ms} and trials lists every
     timing; log (a callable) receives one line per trial."""
@@ -74,14 +75,14 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
     s.set_launch(best["block"] // lanes, best["groups_per_lane"])
     # the persistent engine at the best shape, over the schedules it may
     # prefer (fewer steps matter less to it)
-    if ENGINE_QUEUE in engines:
-        level_best = best
+    level_best = best
+    for engine in (e for e in engines if e != ENGINE_LEVELS):
         for rounds in CHUNK_ROUNDS:
-            s.set_engine(ENGINE_QUEUE, chunk_rounds=rounds)
+            s.set_engine(engine, chunk_rounds=rounds)
             for nb in sorted({1, level_best["batches"]}):
                 s.set_schedule(nb)
                 ms = _time_sweep(ev, s, reps)
-                t = dict(engine=ENGINE_QUEUE, batches=nb,
+                t = dict(engine=engine, batches=nb,
                          block=level_best["block"],
                          groups_per_lane=level_best["groups_per_lane"],
                          chunk_rounds=rounds, ms=ms)

@@ -44,7 +44,7 @@ struct t2_sweeper {
     hipStream_t stream = nullptr;
     int64_t nc = 0, na = 0, ng = 0;
     int precision = T2_FP64, gauss = T2_GAUSS_NOPIVOT;
-    int block = 256, gpl = 1, batches = 1;
+    int block = 256, gpl = 1, batches = 1, order = T2_ORDER_CELL;
     bool use_graph = true;
     t2::HostMesh hm;                 // kept for schedule rebuilds
     std::vector<double> omega_h, sigma_t_h;
@@ -89,9 +89,14 @@ struct t2_sweeper {
     int64_t spin_limit = 0;
     int64_t n_sweeps = 0;            // full sweeps issued
     int64_t n_chunks = -1, queue_grid = 0;
+    int64_t queue_step = -1;         // first queued step (-1: not built)
+    int64_t split = -1;              // hybrid's first queued step (-1: auto)
+    int64_t queue_item = 0;          // first queued item
+    int64_t n_queued = 0;            // queued items
     int64_t *d_chunk_off = nullptr;
-    int32_t *d_pending = nullptr;    // [na][nc]
-    int4 *d_down = nullptr;          // [n_items]
+    int32_t *d_pending = nullptr;    // [n_queued] counters
+    int4 *d_down = nullptr;          // [n_queued] readers
+    int32_t *d_arm = nullptr;        // [n_queued] start values (hybrid)
     unsigned long long *d_ticket = nullptr;
     int32_t *d_abort = nullptr;
 
@@ -176,13 +181,17 @@ constexpr int64_t kDefaultSpinLimit = int64_t(1) << 22;
 
 void free_queue(t2_sweeper *s)
 {
-    const size_t n_items = size_t(s->nc) * s->na;
+    const size_t nq = size_t(s->n_queued);
     drop_graphs(s);
     if (s->n_chunks >= 0)
         dfree(s, s->d_chunk_off, sizeof(int64_t) * size_t(s->n_chunks + 1));
-    dfree(s, s->d_pending, sizeof(int32_t) * n_items);
-    dfree(s, s->d_down, sizeof(int4) * n_items);
+    dfree(s, s->d_pending, sizeof(int32_t) * nq);
+    dfree(s, s->d_down, sizeof(int4) * nq);
+    dfree(s, s->d_arm, sizeof(int32_t) * nq);
     s->n_chunks = -1;
+    s->queue_step = -1;
+    s->queue_item = 0;
+    s->n_queued = 0;
 }
 
 // ------------------------------------------------------------------ sweeps
@@ -205,7 +214,19 @@ int enqueue_sweep(t2_sweeper *s, const double *ang, const void *ang_t)
     a.psi = static_cast<T *>(s->d_psi);
     a.lag_buf = static_cast<const T *>(s->d_lag_buf);
     const auto *geo = static_cast<const t2::ItemGeo<T> *>(s->d_item_geo);
-    if (s->engine == T2_ENGINE_QUEUE) {
+    // per-step launches: the whole schedule (level engine) or the steps
+    // before the queued tail (hybrid)
+    const int64_t level_steps =
+        s->engine == T2_ENGINE_LEVELS ? s->sched.n_steps : s->queue_step;
+    for (int64_t st = 0; st < level_steps; st++) {
+        const int64_t off = s->sched.step_off[st];
+        a.geo = geo + off;
+        a.topo = s->d_item_topo + off;
+        a.n_items = s->sched.step_off[st + 1] - off;
+        T2_HIP(t2::launch_sweep_step<T>(a, int(s->ng), s->gpl, s->block,
+                                        s->stream));
+    }
+    if (s->engine != T2_ENGINE_LEVELS && s->n_chunks > 0) {
         a.geo = geo;
         a.topo = s->d_item_topo;
         a.n_items = s->nc * s->na;
@@ -214,6 +235,8 @@ int enqueue_sweep(t2_sweeper *s, const double *ang, const void *ang_t)
         q.n_chunks = s->n_chunks;
         q.pending = s->d_pending;
         q.down = s->d_down;
+        q.arm = s->d_arm;
+        q.first_item = s->queue_item;
         q.ticket = s->d_ticket;
         q.abort = s->d_abort;
         q.spin_limit = s->spin_limit > 0 ? s->spin_limit : kDefaultSpinLimit;
@@ -222,15 +245,6 @@ int enqueue_sweep(t2_sweeper *s, const double *ang, const void *ang_t)
                                1, s->stream));
         T2_HIP(t2::launch_queue_sweep<T>(a, q, int(s->ng), s->gpl, s->block,
                                          s->queue_grid, s->stream));
-        return T2_OK;
-    }
-    for (int64_t st = 0; st < s->sched.n_steps; st++) {
-        const int64_t off = s->sched.step_off[st];
-        a.geo = geo + off;
-        a.topo = s->d_item_topo + off;
-        a.n_items = s->sched.step_off[st + 1] - off;
-        T2_HIP(t2::launch_sweep_step<T>(a, int(s->ng), s->gpl, s->block,
-                                        s->stream));
     }
     return T2_OK;
 }
@@ -241,29 +255,70 @@ int enqueue_sweep_any(t2_sweeper *s, const double *ang, const void *ang_t)
                                    : enqueue_sweep<float>(s, ang, ang_t);
 }
 
-// Ready-queue tables for the current schedule and launch shape: chunks of
-// one block's items that never span two steps, the readers of every item
-// and the pending-upwind counters.
-int build_queue(t2_sweeper *s)
+int queue_resident(t2_sweeper *s, int64_t *grid)
+{
+    if (s->precision == T2_FP64)
+        T2_HIP(t2::queue_grid<double>(int(s->ng), s->gpl, s->block, grid));
+    else
+        T2_HIP(t2::queue_grid<float>(int(s->ng), s->gpl, s->block, grid));
+    return T2_OK;
+}
+
+// First queued step of the current engine: 0 for the ready queue; for the
+// hybrid, the first step of the thin tail -- the steps after it all hold
+// fewer items than the queue kernel keeps resident in one round, so their
+// per-step launches would leave the GPU mostly idle.
+int queue_first_step(t2_sweeper *s, int64_t *first)
+{
+    *first = 0;
+    if (s->engine != T2_ENGINE_HYBRID)
+        return T2_OK;
+    if (s->split >= 0) {
+        *first = std::min(s->split, s->sched.n_steps);
+        return T2_OK;
+    }
+    int64_t grid = 0;
+    T2_TRY(queue_resident(s, &grid));
+    const int64_t resident_items = grid * (s->block / (s->ng / s->gpl));
+    int64_t st = s->sched.n_steps;
+    while (st > 0 && s->sched.step_off[st] - s->sched.step_off[st - 1] <
+                         resident_items)
+        st--;
+    *first = st;
+    return T2_OK;
+}
+
+// Ready-queue tables for the schedule from step first_step on and the
+// current launch shape: chunks of chunk_rounds rounds of one block's items
+// that never span two steps, the readers of every queued item and the
+// pending-upwind counters.
+int build_queue(t2_sweeper *s, int64_t first_step)
 {
     free_queue(s);
     const int64_t nc = s->nc, na = s->na;
     const int64_t n_items = nc * na;
     if (n_items >= (int64_t(1) << 31))
         return fail(T2_ERR_ARG, "ready queue: more than 2^31 items");
+    const int64_t first = s->sched.step_off[first_step];
+    const int64_t nq = n_items - first;
     const int64_t per_chunk =
         int64_t(s->block / (s->ng / s->gpl)) * s->chunk_rounds;
-    std::vector<int64_t> off(1, 0);
-    for (int64_t st = 0; st < s->sched.n_steps; st++)
+    std::vector<int64_t> off(1, first);
+    for (int64_t st = first_step; st < s->sched.n_steps; st++)
         for (int64_t i = s->sched.step_off[st]; i < s->sched.step_off[st + 1];
              i += per_chunk)
             off.push_back(std::min(i + per_chunk, s->sched.step_off[st + 1]));
-    const int64_t n_chunks = int64_t(off.size()) - 1;
     T2_TRY(dalloc_t(s, &s->d_chunk_off, sizeof(int64_t) * off.size()));
-    s->n_chunks = n_chunks;  // from here on free_queue releases the table
+    // from here on free_queue releases what was allocated
+    s->n_chunks = int64_t(off.size()) - 1;
+    s->queue_step = first_step;
+    s->queue_item = first;
+    s->n_queued = nq;
     T2_TRY(upload(s->d_chunk_off, off.data(), off.size()));
-    T2_TRY(dalloc_t(s, &s->d_pending, sizeof(int32_t) * size_t(n_items)));
-    T2_TRY(dalloc_t(s, &s->d_down, sizeof(int4) * size_t(n_items)));
+    T2_TRY(dalloc_t(s, &s->d_pending, sizeof(int32_t) * size_t(nq)));
+    T2_TRY(dalloc_t(s, &s->d_down, sizeof(int4) * size_t(nq)));
+    if (first > 0)
+        T2_TRY(dalloc_t(s, &s->d_arm, sizeof(int32_t) * size_t(nq)));
     if (!s->d_ticket) {
         T2_TRY(dalloc_t(s, &s->d_ticket, sizeof(unsigned long long)));
         T2_TRY(dalloc_t(s, &s->d_abort, sizeof(int32_t)));
@@ -273,20 +328,21 @@ int build_queue(t2_sweeper *s)
     int *d_err = nullptr;
     auto cleanup = [&]() {
         dfree(s, pos, sizeof(int32_t) * size_t(n_items));
-        dfree(s, dcount, sizeof(int32_t) * size_t(n_items));
+        dfree(s, dcount, sizeof(int32_t) * size_t(nq));
         dfree(s, d_err, sizeof(int));
     };
     int rc = T2_OK, err = 0;
     if ((rc = dalloc_t(s, &pos, sizeof(int32_t) * size_t(n_items))) ||
-        (rc = dalloc_t(s, &dcount, sizeof(int32_t) * size_t(n_items))) ||
+        (rc = dalloc_t(s, &dcount, sizeof(int32_t) * size_t(nq))) ||
         (rc = dalloc_t(s, &d_err, sizeof(int)))) {
         cleanup();
         return rc;
     }
     hipError_t e = hipMemsetAsync(d_err, 0, sizeof(int), s->stream);
     if (e == hipSuccess)
-        e = t2::launch_queue_links(s->d_item_topo, n_items, nc, pos, dcount,
-                                   s->d_down, s->d_pending, d_err, s->stream);
+        e = t2::launch_queue_links(s->d_item_topo, n_items, nc, first, pos,
+                                   dcount, s->d_down, s->d_pending, s->d_arm,
+                                   d_err, s->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost,
                            s->stream);
@@ -299,20 +355,27 @@ int build_queue(t2_sweeper *s)
     if (err)
         return fail(T2_ERR_ARG, "ready-queue set-up: a cell feeds more than "
                                 "four cells of one angle");
-    int64_t grid = 0;
-    if (s->precision == T2_FP64)
-        T2_HIP(t2::queue_grid<double>(int(s->ng), s->gpl, s->block, &grid));
-    else
-        T2_HIP(t2::queue_grid<float>(int(s->ng), s->gpl, s->block, &grid));
-    s->queue_grid = grid;
-    return T2_OK;
+    return queue_resident(s, &s->queue_grid);
+}
+
+// Tables of the current engine, rebuilt when stale.
+int ensure_queue(t2_sweeper *s)
+{
+    if (s->engine == T2_ENGINE_LEVELS)
+        return T2_OK;
+    int64_t first = 0;
+    T2_TRY(queue_first_step(s, &first));
+    if (s->n_chunks >= 0 && s->queue_step == first)
+        return T2_OK;
+    T2_HIP(hipStreamSynchronize(s->stream));
+    return build_queue(s, first);
 }
 
 // After an abandoned ready-queue sweep: reports it once and re-arms the
 // counters so that later sweeps are valid.
 int check_abort(t2_sweeper *s)
 {
-    if (s->engine != T2_ENGINE_QUEUE || !s->d_abort)
+    if (!s->d_abort)
         return T2_OK;
     int32_t flag = 0;
     T2_HIP(hipStreamSynchronize(s->stream));
@@ -320,8 +383,9 @@ int check_abort(t2_sweeper *s)
     if (!flag)
         return T2_OK;
     if (s->d_pending)
-        T2_HIP(t2::launch_queue_rearm(s->d_item_topo, s->nc * s->na, s->nc,
-                                      s->d_pending, s->stream));
+        T2_HIP(t2::launch_queue_rearm(s->d_item_topo, s->nc * s->na,
+                                      s->queue_item, s->d_arm, s->d_pending,
+                                      s->stream));
     T2_HIP(hipMemsetAsync(s->d_abort, 0, sizeof(int32_t), s->stream));
     T2_HIP(hipStreamSynchronize(s->stream));
     return fail(T2_ERR_TIMEOUT,
@@ -341,8 +405,7 @@ int after_queue(t2_sweeper *s, int rc)
 // per source combination into a hipGraph and replayed.
 int run_sweep(t2_sweeper *s, const double *ang, const void *ang_t = nullptr)
 {
-    if (s->engine == T2_ENGINE_QUEUE && s->n_chunks < 0)
-        T2_TRY(build_queue(s));
+    T2_TRY(ensure_queue(s));
     s->n_sweeps++;
     if (!s->use_graph)
         return enqueue_sweep_any(s, ang, ang_t);
@@ -488,6 +551,7 @@ int build_schedule_device(t2_sweeper *s)
     const int64_t nc = s->nc, na = s->na, ng = s->ng;
     free_schedule(s);
     T2_TRY(t2::build_schedule(s->hm, na, s->omega_h.data(), s->batches,
+                              s->order,
                               s->sched));
     const size_t n_items = size_t(nc) * na;
     std::vector<double> geo(size_t(nc) * t2::kGeoStride, 0.0);
@@ -858,7 +922,7 @@ int t2_sweeper_info(const t2_sweeper *s, t2_info *info)
     info->items_per_block = int32_t(s->block / (s->ng / s->gpl));
     info->groups_per_lane = s->gpl;
     info->chunk_rounds = s->chunk_rounds;
-    info->reserved = 0;
+    info->queue_first_step = int32_t(s->queue_step);
     return T2_OK;
 }
 
@@ -889,6 +953,20 @@ int t2_set_launch(t2_sweeper *s, int items_per_block, int groups_per_lane)
     s->block = block;
     s->gpl = gpl;
     return T2_OK;
+}
+
+int t2_set_order(t2_sweeper *s, int order)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (order != T2_ORDER_CELL && order != T2_ORDER_FACES)
+        return fail(T2_ERR_ARG, "order must be T2_ORDER_CELL or "
+                                "T2_ORDER_FACES");
+    if (order == s->order)
+        return T2_OK;
+    T2_HIP(hipStreamSynchronize(s->stream));
+    s->order = order;
+    return build_schedule_device(s);
 }
 
 int t2_set_schedule(t2_sweeper *s, int angle_batches)
@@ -1208,9 +1286,10 @@ int t2_set_engine(t2_sweeper *s, int engine, int chunk_rounds,
 {
     t2::clear_error();
     T2_TRY(check(s));
-    if (engine != T2_ENGINE_LEVELS && engine != T2_ENGINE_QUEUE)
-        return fail(T2_ERR_ARG, "engine must be T2_ENGINE_LEVELS or "
-                                "T2_ENGINE_QUEUE");
+    if (engine != T2_ENGINE_LEVELS && engine != T2_ENGINE_QUEUE &&
+        engine != T2_ENGINE_HYBRID)
+        return fail(T2_ERR_ARG, "engine must be T2_ENGINE_LEVELS, "
+                                "T2_ENGINE_QUEUE or T2_ENGINE_HYBRID");
     if (spin_limit < 0)
         return fail(T2_ERR_ARG, "spin_limit < 0");
     if (chunk_rounds < 0 || chunk_rounds > 256)
@@ -1224,6 +1303,20 @@ int t2_set_engine(t2_sweeper *s, int engine, int chunk_rounds,
     s->engine = engine;
     s->chunk_rounds = rounds;
     s->spin_limit = spin_limit;
+    return T2_OK;
+}
+
+int t2_set_queue_split(t2_sweeper *s, int64_t first_step)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (first_step < -1)
+        return fail(T2_ERR_ARG, "first_step must be >= -1");
+    if (first_step != s->split) {
+        T2_HIP(hipStreamSynchronize(s->stream));
+        drop_graphs(s);
+    }
+    s->split = first_step;
     return T2_OK;
 }
 
@@ -1478,8 +1571,10 @@ int build_batch_queue(t2_batch *b)
     std::vector<t2::BatchProblem<T>> probs(nm);
     for (int i = 0; i < nm; i++) {
         t2_sweeper *m = b->members[i];
-        if (m->n_chunks < 0)
-            T2_TRY(build_queue(m));
+        if (m->n_chunks < 0 || m->queue_step != 0) {
+            T2_HIP(hipStreamSynchronize(m->stream));
+            T2_TRY(build_queue(m, 0));
+        }
         t2::BatchProblem<T> &bp = probs[i];
         bp.a = t2::SweepArgs<T>{};
         bp.a.geo = static_cast<const t2::ItemGeo<T> *>(m->d_item_geo);
@@ -1493,6 +1588,8 @@ int build_batch_queue(t2_batch *b)
         bp.a.lag_buf = static_cast<const T *>(m->d_lag_buf);
         bp.pending = m->d_pending;
         bp.down = m->d_down;
+        bp.arm = m->d_arm;
+        bp.first_item = m->queue_item;
         bp.groups = int32_t(m->ng);
     }
     std::vector<t2::BatchChunk> chunks;
@@ -1642,7 +1739,8 @@ int batch_check_abort(t2_batch *b)
     for (auto *m : b->members)
         if (m->d_pending)
             T2_HIP(t2::launch_queue_rearm(m->d_item_topo, m->nc * m->na,
-                                          m->nc, m->d_pending, b->stream));
+                                          m->queue_item, m->d_arm,
+                                          m->d_pending, b->stream));
     T2_HIP(hipMemsetAsync(b->d_abort, 0, sizeof(int32_t), b->stream));
     T2_HIP(hipStreamSynchronize(b->stream));
     return fail(T2_ERR_TIMEOUT,

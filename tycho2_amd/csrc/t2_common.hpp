@@ -74,7 +74,10 @@ int build_levels(const HostMesh &m, int64_t na, const double *omega,
 std::vector<int32_t> angle_batches(int64_t na, const double *omega,
                                    int batches);
 
+// Items in step order: by angle batch and level, then within a step by
+// angle and, for order T2_ORDER_FACES, the pattern of incoming interior
+// faces, then by cell.
 int build_schedule(const HostMesh &m, int64_t na, const double *omega,
-                   int batches, Schedule &s);
+                   int batches, int order, Schedule &s);
 
 }  // namespace t2

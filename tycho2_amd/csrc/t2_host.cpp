@@ -2,6 +2,7 @@
 // schedules.  Pure C++ (no device calls); built with -ffp-contract=off so the
 // floating-point results match the reference's TychoMesh bit for bit.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <thread>
@@ -293,7 +294,7 @@ std::vector<int32_t> angle_batches(int64_t na, const double *omega,
 // L_b the most levels of any of its angles; step off_b + l holds every
 // (angle, cell) of the batch at level l, angle-major and cell-ascending.
 int build_schedule(const HostMesh &m, int64_t na, const double *omega,
-                   int batches, Schedule &s)
+                   int batches, int order, Schedule &s)
 {
     const int64_t nc = m.nc;
     int cb = 1;
@@ -343,11 +344,36 @@ int build_schedule(const HostMesh &m, int64_t na, const double *omega,
         s.step_off[st + 1] += s.step_off[st];
     std::vector<int64_t> pos(s.step_off.begin(), s.step_off.end() - 1);
     s.work.resize(size_t(na) * nc);
+    std::vector<int32_t> cells(nc);
     for (int64_t a = 0; a < na; a++) {
         const int32_t *l = levels.data() + a * nc;
         for (int64_t c = 0; c < nc; c++)
+            cells[c] = int32_t(c);
+        if (order == T2_ORDER_FACES) {
+            // stable counting sort of the cells by incoming-face pattern, so
+            // that the items sharing a wavefront take the same upwind faces
+            const double *om = omega + 3 * a;
+            std::vector<uint8_t> mask(nc);
+            std::array<int64_t, 17> start{};
+            for (int64_t c = 0; c < nc; c++) {
+                unsigned mk = 0;
+                for (int k = 0; k < 4; k++)
+                    if (m.adj[4 * c + k] >= 0 &&
+                        omega_dot_n(om, &m.normal[(4 * c + k) * 3]) < 0)
+                        mk |= 1u << k;
+                mask[c] = uint8_t(mk);
+                start[mk + 1]++;
+            }
+            for (int b = 0; b < 16; b++)
+                start[b + 1] += start[b];
+            for (int64_t c = 0; c < nc; c++)
+                cells[start[mask[c]]++] = int32_t(c);
+        }
+        for (int64_t i = 0; i < nc; i++) {
+            const int64_t c = cells[i];
             s.work[pos[boff[batch[a]] + l[c]]++] =
                 uint32_t(c) | (uint32_t(a) << cb);
+        }
     }
     return T2_OK;
 }

@@ -326,7 +326,9 @@ sweep_step_kernel(const SweepArgs<T> p)
 template <typename T, int G, int GPL>
 __device__ __forceinline__ void queue_chunk(const SweepArgs<T> &p,
                                             int32_t *pending,
-                                            const int4 *down, int64_t lo,
+                                            const int4 *down,
+                                            const int32_t *arm,
+                                            int64_t first_item, int64_t lo,
                                             int64_t hi, const QueueState &q)
 {
     constexpr int LPI = G / GPL;          // lanes per item
@@ -342,8 +344,7 @@ __device__ __forceinline__ void queue_chunk(const SweepArgs<T> &p,
     const int64_t first = w_lo + lane / LPI;
     bool ok = true;
     for (int64_t it = first; ok && it < w_hi; it += IPR) {
-        const ItemTopo t = p.topo[it];
-        const int32_t *mine = pending + size_t(t.angle) * size_t(p.nc) + t.cell;
+        const int32_t *mine = pending + (it - first_item);
         int64_t spins = 0;
         while (__hip_atomic_load(mine, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT) > 0) {
@@ -374,19 +375,24 @@ __device__ __forceinline__ void queue_chunk(const SweepArgs<T> &p,
     __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     if (ok && gl == 0) {
         for (int64_t it = first; it < w_hi; it += IPR) {
-            const ItemTopo t = p.topo[it];
-            int32_t *row = pending + size_t(t.angle) * size_t(p.nc);
-            const int n_up = (t.up[0] >= 0) + (t.up[1] >= 0) +
-                             (t.up[2] >= 0) + (t.up[3] >= 0);
+            int n_up;
+            if (arm) {
+                n_up = arm[it - first_item];
+            } else {
+                const ItemTopo t = p.topo[it];
+                n_up = (t.up[0] >= 0) + (t.up[1] >= 0) + (t.up[2] >= 0) +
+                       (t.up[3] >= 0);
+            }
             // re-armed for the next sweep (nothing else touches it now)
-            __hip_atomic_store(row + t.cell, n_up, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            const int4 d = down[it];
+            __hip_atomic_store(pending + (it - first_item), n_up,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int4 d = down[it - first_item];
             const int32_t dn[4] = {d.x, d.y, d.z, d.w};
 #pragma unroll
             for (int k = 0; k < 4; k++)
                 if (dn[k] >= 0)
-                    __hip_atomic_fetch_add(row + dn[k], -1, __ATOMIC_RELAXED,
+                    __hip_atomic_fetch_add(pending + (dn[k] - first_item), -1,
+                                           __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
         }
     }
@@ -420,8 +426,8 @@ queue_sweep_kernel(const SweepArgs<T> p, const QueueState q)
         const int64_t c = next_chunk(q, &s_chunk);
         if (c >= q.n_chunks)
             break;
-        queue_chunk<T, G, GPL>(p, q.pending, q.down, q.chunk_off[c],
-                               q.chunk_off[c + 1], q);
+        queue_chunk<T, G, GPL>(p, q.pending, q.down, q.arm, q.first_item,
+                               q.chunk_off[c], q.chunk_off[c + 1], q);
     }
 }
 
@@ -442,24 +448,23 @@ queue_batch_kernel(const BatchProblem<T> *problems, const BatchChunk *chunks,
         const BatchProblem<T> &bp = problems[ch.problem];
         const int64_t lo = ch.lo, hi = ch.lo + ch.n;
         switch (bp.groups) {
-        case 1: queue_chunk<T, 1, 1>(bp.a, bp.pending, bp.down, lo, hi, q); break;
-        case 2: queue_chunk<T, 2, 1>(bp.a, bp.pending, bp.down, lo, hi, q); break;
-        case 4: queue_chunk<T, 4, 1>(bp.a, bp.pending, bp.down, lo, hi, q); break;
-        case 8: queue_chunk<T, 8, 1>(bp.a, bp.pending, bp.down, lo, hi, q); break;
-        case 16: queue_chunk<T, 16, 1>(bp.a, bp.pending, bp.down, lo, hi, q); break;
-        case 32: queue_chunk<T, 32, 1>(bp.a, bp.pending, bp.down, lo, hi, q); break;
-        case 64: queue_chunk<T, 64, 1>(bp.a, bp.pending, bp.down, lo, hi, q); break;
+        case 1: queue_chunk<T, 1, 1>(bp.a, bp.pending, bp.down, bp.arm, bp.first_item, lo, hi, q); break;
+        case 2: queue_chunk<T, 2, 1>(bp.a, bp.pending, bp.down, bp.arm, bp.first_item, lo, hi, q); break;
+        case 4: queue_chunk<T, 4, 1>(bp.a, bp.pending, bp.down, bp.arm, bp.first_item, lo, hi, q); break;
+        case 8: queue_chunk<T, 8, 1>(bp.a, bp.pending, bp.down, bp.arm, bp.first_item, lo, hi, q); break;
+        case 16: queue_chunk<T, 16, 1>(bp.a, bp.pending, bp.down, bp.arm, bp.first_item, lo, hi, q); break;
+        case 32: queue_chunk<T, 32, 1>(bp.a, bp.pending, bp.down, bp.arm, bp.first_item, lo, hi, q); break;
+        case 64: queue_chunk<T, 64, 1>(bp.a, bp.pending, bp.down, bp.arm, bp.first_item, lo, hi, q); break;
         default: break;
         }
     }
 }
 
-// pos[a][c] = item of (a, c); pending[a][c] = its interior upwind count;
-// down[] cleared.
+// pos[a][c] = item of (a, c) for every item; the queued items' readers
+// tables cleared.
 __global__ void queue_index_kernel(const ItemTopo *__restrict__ topo,
-                                   int64_t n, int64_t nc,
+                                   int64_t n, int64_t nc, int64_t first_item,
                                    int32_t *__restrict__ pos,
-                                   int32_t *__restrict__ pending,
                                    int32_t *__restrict__ dcount,
                                    int4 *__restrict__ down)
 {
@@ -467,39 +472,69 @@ __global__ void queue_index_kernel(const ItemTopo *__restrict__ topo,
     if (i >= n)
         return;
     const ItemTopo t = topo[i];
-    const size_t idx = size_t(t.angle) * size_t(nc) + t.cell;
-    if (pos)
-        pos[idx] = int32_t(i);
-    pending[idx] = (t.up[0] >= 0) + (t.up[1] >= 0) + (t.up[2] >= 0) +
-                   (t.up[3] >= 0);
-    if (dcount) {
-        dcount[i] = 0;
-        down[i] = make_int4(-1, -1, -1, -1);
+    pos[size_t(t.angle) * size_t(nc) + t.cell] = int32_t(i);
+    if (i >= first_item) {
+        dcount[i - first_item] = 0;
+        down[i - first_item] = make_int4(-1, -1, -1, -1);
     }
 }
 
-// down[item of (a, u)] collects every cell c of angle a that reads u's psi.
+// Counters of the queued items back to their start values.
+__global__ void queue_rearm_kernel(const ItemTopo *__restrict__ topo,
+                                   int64_t n, int64_t first_item,
+                                   const int32_t *__restrict__ arm,
+                                   int32_t *__restrict__ pending)
+{
+    const int64_t i = first_item + int64_t(blockIdx.x) * blockDim.x +
+                      threadIdx.x;
+    if (i >= n)
+        return;
+    int v;
+    if (arm) {
+        v = arm[i - first_item];
+    } else {
+        const ItemTopo t = topo[i];
+        v = (t.up[0] >= 0) + (t.up[1] >= 0) + (t.up[2] >= 0) + (t.up[3] >= 0);
+    }
+    pending[i - first_item] = v;
+}
+
+// Links the queued items: down[j - first_item] collects the items that read
+// item j's psi, and each queued item's counter (and arm[] copy, if given)
+// is the number of its upwind items that are queued too (the others finished
+// in earlier launches).
 __global__ void queue_down_kernel(const ItemTopo *__restrict__ topo,
-                                  int64_t n, int64_t nc,
+                                  int64_t n, int64_t nc, int64_t first_item,
                                   const int32_t *__restrict__ pos,
                                   int32_t *__restrict__ dcount,
-                                  int4 *__restrict__ down, int *err)
+                                  int4 *__restrict__ down,
+                                  int32_t *__restrict__ pending,
+                                  int32_t *__restrict__ arm, int *err)
 {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t i = first_item + int64_t(blockIdx.x) * blockDim.x +
+                      threadIdx.x;
     if (i >= n)
         return;
     const ItemTopo t = topo[i];
+    int count = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         if (t.up[k] < 0)
             continue;
         const int32_t j = pos[size_t(t.angle) * size_t(nc) + t.up[k]];
-        const int slot = atomicAdd(dcount + j, 1);
+        if (j < first_item)
+            continue;
+        count++;
+        const int slot = atomicAdd(dcount + (j - first_item), 1);
         if (slot < 4)
-            reinterpret_cast<int32_t *>(down + j)[slot] = t.cell;
+            reinterpret_cast<int32_t *>(down + (j - first_item))[slot] =
+                int32_t(i);
         else
             *err = 1;
     }
+    pending[i - first_item] = count;
+    if (arm)
+        arm[i - first_item] = count;
 }
 
 template <typename T, int G>
@@ -1038,31 +1073,32 @@ hipError_t queue_batch_grid(int64_t *grid)
 }
 
 hipError_t launch_queue_links(const ItemTopo *topo, int64_t n_items,
-                              int64_t nc, int32_t *pos, int32_t *dcount,
-                              int4 *down, int32_t *pending, int *err,
-                              hipStream_t stream)
+                              int64_t nc, int64_t first_item, int32_t *pos,
+                              int32_t *dcount, int4 *down, int32_t *pending,
+                              int32_t *arm, int *err, hipStream_t stream)
 {
-    if (n_items <= 0)
+    if (n_items <= first_item)
         return hipSuccess;
-    const unsigned grid = unsigned((n_items + 255) / 256);
-    hipLaunchKernelGGL(queue_index_kernel, dim3(grid), dim3(256), 0, stream,
-                       topo, n_items, nc, pos, pending, dcount, down);
-    hipLaunchKernelGGL(queue_down_kernel, dim3(grid), dim3(256), 0, stream,
-                       topo, n_items, nc, pos, dcount, down, err);
+    hipLaunchKernelGGL(queue_index_kernel, dim3(unsigned((n_items + 255) / 256)),
+                       dim3(256), 0, stream, topo, n_items, nc, first_item,
+                       pos, dcount, down);
+    hipLaunchKernelGGL(queue_down_kernel,
+                       dim3(unsigned((n_items - first_item + 255) / 256)),
+                       dim3(256), 0, stream, topo, n_items, nc, first_item,
+                       pos, dcount, down, pending, arm, err);
     return hipGetLastError();
 }
 
 hipError_t launch_queue_rearm(const ItemTopo *topo, int64_t n_items,
-                              int64_t nc, int32_t *pending,
-                              hipStream_t stream)
+                              int64_t first_item, const int32_t *arm,
+                              int32_t *pending, hipStream_t stream)
 {
-    if (n_items <= 0)
+    if (n_items <= first_item)
         return hipSuccess;
-    hipLaunchKernelGGL(queue_index_kernel, dim3(unsigned((n_items + 255) / 256)),
-                       dim3(256), 0, stream, topo, n_items, nc,
-                       static_cast<int32_t *>(nullptr), pending,
-                       static_cast<int32_t *>(nullptr),
-                       static_cast<int4 *>(nullptr));
+    hipLaunchKernelGGL(queue_rearm_kernel,
+                       dim3(unsigned((n_items - first_item + 255) / 256)),
+                       dim3(256), 0, stream, topo, n_items, first_item, arm,
+                       pending);
     return hipGetLastError();
 }
 

@@ -127,18 +127,21 @@ hipError_t launch_sweep_step(const SweepArgs<T> &args, int groups,
                              hipStream_t stream);
 
 // Ready-queue form of a sweep: one persistent launch over the step-ordered
-// items, split into chunks that never span two steps.
+// items from first_item on, split into chunks that never span two steps.
+// The queued items' tables are indexed by item - first_item.
 struct QueueState {
     const int64_t *chunk_off;      // [n_chunks + 1] item ranges
     int64_t n_chunks;
-    int32_t *pending;              // [na][nc] upwind cells not yet solved
-    const int4 *down;              // [n_items] cells reading the item's psi
+    int32_t *pending;              // upwind queued items not yet solved
+    const int4 *down;              // items reading the item's psi (-1 none)
+    const int32_t *arm;            // counters' start values; null: all
+                                   // upwind interior faces (first_item 0)
+    int64_t first_item;
     unsigned long long *ticket;    // next chunk to hand out (0 at the start)
     int32_t *abort;                // set when a wait exceeds spin_limit
     int64_t spin_limit;            // polls per wait before giving up
 };
 
-// grid: blocks to launch (queue_grid gives the number resident at once).
 template <typename T>
 hipError_t launch_queue_sweep(const SweepArgs<T> &args, const QueueState &q,
                               int groups, int groups_per_lane, int block,
@@ -152,8 +155,10 @@ hipError_t queue_grid(int groups, int groups_per_lane, int block,
 template <typename T>
 struct BatchProblem {
     SweepArgs<T> a;                // records (whole sweep), arrays, nc
-    int32_t *pending;              // the problem's [na][nc] counters
-    const int4 *down;              // the problem's readers table
+    int32_t *pending;              // the problem's tables, as in QueueState
+    const int4 *down;
+    const int32_t *arm;
+    int64_t first_item;
     int32_t groups;                // 1, 2, 4, ..., 64
 };
 struct BatchChunk {
@@ -168,16 +173,18 @@ hipError_t launch_queue_batch(const BatchProblem<T> *problems,
 template <typename T>
 hipError_t queue_batch_grid(int64_t *grid);
 
-// Builds down[] and pending[] from the item records (pos and dcount: scratch
-// of na * nc and n_items int32); *err is set if an item had > 4 readers.
+// Links the queued items >= first_item: down[] and pending[] (and arm[],
+// if not null) from the item records; pos (na * nc) and dcount (queued
+// items) are int32 scratch; *err is set if an item had more than 4 readers.
 hipError_t launch_queue_links(const ItemTopo *topo, int64_t n_items,
-                              int64_t nc, int32_t *pos, int32_t *dcount,
-                              int4 *down, int32_t *pending, int *err,
-                              hipStream_t stream);
-// Resets pending[] to the upwind counts (after an aborted sweep).
+                              int64_t nc, int64_t first_item, int32_t *pos,
+                              int32_t *dcount, int4 *down, int32_t *pending,
+                              int32_t *arm, int *err, hipStream_t stream);
+// Resets the queued items' counters to their start values (after an
+// abandoned sweep).
 hipError_t launch_queue_rearm(const ItemTopo *topo, int64_t n_items,
-                              int64_t nc, int32_t *pending,
-                              hipStream_t stream);
+                              int64_t first_item, const int32_t *arm,
+                              int32_t *pending, hipStream_t stream);
 
 // lag_buf[i][j][g] = psi[a_i][u_i][vertex of u_i facing c_i's vertex j][g]
 template <typename T>

@@ -23,6 +23,9 @@ FP32 = 1
 GAUSS_NOPIVOT = 1
 ENGINE_LEVELS = 0
 ENGINE_QUEUE = 1
+ENGINE_HYBRID = 2
+ORDER_CELL = 0
+ORDER_FACES = 1
 ERR_TIMEOUT = -7
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -47,8 +50,10 @@ SIGNATURES = {
     "t2_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
                                      ctypes.c_int]),
     "t2_set_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "t2_set_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "t2_set_engine": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int64]),
+    "t2_set_queue_split": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
     "t2_solve_refined": (ctypes.c_int, [
         ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
         ctypes.c_int64, ctypes.c_void_p, _dp]),
@@ -98,7 +103,7 @@ class T2Info(ctypes.Structure):
                 ("items_per_block", ctypes.c_int32),
                 ("groups_per_lane", ctypes.c_int32),
                 ("chunk_rounds", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("queue_first_step", ctypes.c_int32)]
 
 
 class T2RefineInfo(ctypes.Structure):
@@ -243,6 +248,10 @@ class SweeperHIP:
     def set_schedule(self, angle_batches: int) -> None:
         _check(lib().t2_set_schedule(self.handle, int(angle_batches)))
 
+    def set_order(self, order: int) -> None:
+        """ORDER_CELL or ORDER_FACES: item order within a step."""
+        _check(lib().t2_set_order(self.handle, int(order)))
+
     def set_engine(self, engine: int, spin_limit: int = 0,
                    chunk_rounds: int = 0) -> None:
         """ENGINE_LEVELS (per-step launches in a hipGraph) or ENGINE_QUEUE
@@ -251,6 +260,11 @@ class SweeperHIP:
         defaults)."""
         _check(lib().t2_set_engine(self.handle, int(engine), int(chunk_rounds),
                                    int(spin_limit)))
+
+    def set_queue_split(self, first_step: int = -1) -> None:
+        """ENGINE_HYBRID: first step run by the ready queue (-1: the thin
+        tail, chosen by the library)."""
+        _check(lib().t2_set_queue_split(self.handle, int(first_step)))
 
     # -- SweeperAbstract::sweep
     def sweep(self, psi: np.ndarray, source: np.ndarray,
