@@ -63,7 +63,11 @@ def path_leg(p, precision, steps, warmup, ev, tune, solve):
     log(f"precision {precision}: handle ready in {out['setup_s']}s")
     try:
         if tune:
-            best, _ = autotune.tune(s, reps=2, log=log)
+            # angle batches > 2 and the pure queue never win at this size
+            # (measured); the tuner still checks the launch shapes and the
+            # hybrid's queued tail
+            best, _ = autotune.tune(s, reps=2, batches=(1, 2),
+                                    engines=(0, 2), log=log)
             out["tuned"] = best
             log(f"autotune: {best}")
         info = s.info()
@@ -80,34 +84,43 @@ def path_leg(p, precision, steps, warmup, ev, tune, solve):
         out["psi_gb"] = info["psi_bytes"] / 1e9
         if solve:
             cfg = p.solver
-            sweeps0 = s.info()["sweeps"]
+            holder = {}
             if precision == FP32:
-                holder = {}
+                # untimed first solve: device allocations, and phi after the
+                # FP32 stage for the error report
+                holder["info"], holder["phi1"] = s.solve_refined(
+                    cfg["iter_max"], 1e-6, 1e-5, cfg["restart"],
+                    want_phi1=True)
+                out["phi1"] = holder["phi1"]
+                sweeps0 = s.info()["sweeps"]
 
                 def run():
-                    holder["info"], holder["phi1"] = s.solve_refined(
-                        cfg["iter_max"], 1e-6, 1e-5, cfg["restart"],
-                        want_phi1=True)
+                    holder["info"], _ = s.solve_refined(
+                        cfg["iter_max"], 1e-6, 1e-5, cfg["restart"])
                 ms = time_region(ev, s, run, 1)
-                out["solve_ms"] = ms
                 out["refine"] = holder["info"]
-                out["phi1"] = holder["phi1"]
             else:
-                holder = {}
+                s.solve_krylov(cfg["iter_max"], cfg["tol"], cfg["restart"])
+                sweeps0 = s.info()["sweeps"]
 
                 def run():
                     holder["hist"] = s.solve_krylov(cfg["iter_max"],
                                                     cfg["tol"],
                                                     cfg["restart"])
                 ms = time_region(ev, s, run, 1)
-                out["solve_ms"] = ms
                 out["iters"] = len(holder["hist"])
                 out["residual"] = float(holder["hist"][-1]) \
                     if len(holder["hist"]) else 0.0
+            out["solve_ms"] = ms
             out["sweeps"] = s.info()["sweeps"] - sweeps0
+            out["phi"] = s.phi()
             log(f"precision {precision}: solve {out['solve_ms']:.1f} ms, "
                 f"{out['sweeps']} sweeps")
-            out["phi"] = s.phi()
+            if precision != FP32:
+                # the reference for every error: FP64 GMRES far below 1e-10
+                s.solve_krylov(cfg["iter_max"], 1e-13, cfg["restart"])
+                out["phi_ref"] = s.phi()
+                log("reference phi: FP64 GMRES to 1e-13")
             _ = np
     finally:
         s.close()
@@ -201,7 +214,8 @@ def main() -> int:
     solve = not args.no_solve
     tune = not args.no_tune
     r64 = path_leg(p, FP64, args.steps, args.warmup, ev, tune, solve)
-    log("fp64: " + str({k: v for k, v in r64.items() if k != "phi"}))
+    log("fp64: " + str({k: v for k, v in r64.items()
+                        if k not in ("phi", "phi_ref")}))
     r32 = path_leg(p, FP32, args.steps, args.warmup, ev, tune, solve)
     log("fp32: " + str({k: v for k, v in r32.items()
                         if k not in ("phi", "phi1")}))
@@ -257,8 +271,12 @@ def main() -> int:
                      gmres_iters=r64["iters"], sweeps=r64["sweeps"],
                      solves_per_s_solve=round(r64["sweeps"] * solves /
                                               (r64["solve_ms"] * 1e-3)))
-        e1 = maxrel(r32["phi1"], r64["phi"])
-        e2 = maxrel(r32["phi"], r64["phi"])
+        ref = r64["phi_ref"]
+        e1 = maxrel(r32["phi1"], ref)
+        e2 = maxrel(r32["phi"], ref)
+        plain["phi_maxrel_err"] = maxrel(r64["phi"], ref)
+        plain["error_reference"] = ("FP64 GMRES(20) to 1e-13 on the same "
+                                    "handle (untimed)")
         refined.update(solve_s=round(r32["solve_ms"] / 1e3, 3),
                        sweeps=r32["sweeps"],
                        solves_per_s_solve=round(r32["sweeps"] * solves /
@@ -274,7 +292,8 @@ def main() -> int:
         if e2 <= 1e-10:
             value = refined["solves_per_s_solve"]
             metric_path = ("FP32 sweeps + FP64 defect correction, whole "
-                           "solve to 1e-10 (max-rel phi err vs FP64)")
+                           "solve to <=1e-10 max-rel phi err vs FP64 GMRES "
+                           "to 1e-13")
             ms_per_step = r32["solve_ms"] / r32["sweeps"]
             dtype = "f32+f64"
             roofline = refined["roofline"]

@@ -128,7 +128,7 @@ int t2_set_launch(t2_sweeper *s, int items_per_block, int groups_per_lane);
 int t2_set_schedule(t2_sweeper *s, int angle_batches);
 
 /* Order of the items within a step (any order is valid; results do not
- * depend on it): T2_ORDER_CELL by angle then cell (default); T2_ORDER_FACES
+ * depend on it): T2_ORDER_CELL by angle then cell; T2_ORDER_FACES (default)
  * by angle, then the cell's pattern of incoming interior faces, then cell,
  * so that the items sharing a wavefront read the same faces. */
 #define T2_ORDER_CELL 0
@@ -197,8 +197,10 @@ int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,
  *   2. r = q + M S phi1 - L psi32, the defect of the coupled equations,
  *      evaluated in FP64 from FP64 cell coefficients (a parallel operator
  *      apply, no sweep), stored in FP32 as a nodal source density.
- *   3. GMRES on the correction (L - M S D) delta = r with FP32 sweeps, to
- *      rtol_correction; phi = phi1 + D delta in FP64.
+ *   3. GMRES on the correction (I - D L^-1 M S) D delta = D L^-1 r with FP32
+ *      sweeps, to rtol_correction; phi = phi1 + D delta in FP64.
+ *   4. psi = L^-1 (q + M S phi) by one FP32 sweep (the angular flux that
+ *      goes with phi, to FP32 accuracy).
  * Memory: psi32 plus an FP32 residual (psi-sized each).  phi1 (host,
  * [cell][vertex][group], may be NULL) receives the scalar flux after step 1
  * so callers can report the error of each step. */
@@ -208,7 +210,7 @@ typedef struct {
     int64_t sweeps;            /* FP32 sweeps in total                    */
     double residual_sweep;     /* relative GMRES residual after step 1    */
     double residual_correction;
-    double correction_rel;     /* max|D delta| / max|phi|                 */
+    double correction_rel;     /* ||D delta||_2 / ||phi||_2               */
 } t2_refine_info;
 
 int t2_solve_refined(t2_sweeper *s, int64_t iter_max, double rtol_sweep,

@@ -153,15 +153,14 @@ def test_queue_wait_limit_abandons_and_recovers():
 @pytest.mark.parametrize("engine", [ENGINE_LEVELS, ENGINE_QUEUE])
 def test_face_order_bitwise(engine):
     """Items sorted by incoming-face pattern within a step: same results."""
-    from tycho2_amd.sweeper import ORDER_FACES
+    from tycho2_amd.sweeper import ORDER_CELL, ORDER_FACES
     print(f"face order engine={engine}", flush=True)
     sm, conn, omega, wt, st, ss, q = problem(5, 4, 16, seed=90)
     out = []
-    for order in (None, ORDER_FACES):
+    for order in (ORDER_CELL, ORDER_FACES):
         with SweeperHIP(sm, conn, omega, wt, 16, st, ss, FP64) as s:
             s.set_engine(engine, SPIN)
-            if order is not None:
-                s.set_order(order)
+            s.set_order(order)
             s.set_source_isotropic(q)
             for _ in range(2):
                 s.sweep_step()

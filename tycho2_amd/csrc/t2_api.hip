@@ -44,7 +44,7 @@ struct t2_sweeper {
     hipStream_t stream = nullptr;
     int64_t nc = 0, na = 0, ng = 0;
     int precision = T2_FP64, gauss = T2_GAUSS_NOPIVOT;
-    int block = 256, gpl = 1, batches = 1, order = T2_ORDER_CELL;
+    int block = 256, gpl = 1, batches = 1, order = T2_ORDER_FACES;
     bool use_graph = true;
     t2::HostMesh hm;                 // kept for schedule rebuilds
     std::vector<double> omega_h, sigma_t_h;
@@ -727,9 +727,16 @@ int gmres(t2_sweeper *s, const double *b, double *x, int64_t m, double rtol,
         y(m), h(m + 1), h2(m + 1);
     int64_t it = 0;
     double rnorm = bnorm;
+    bool x_zero = true;
     while (it < iter_max && rnorm > target) {
-        T2_TRY(transport_apply(s, x, V));
-        T2_HIP(t2::launch_axpby(1.0, b, -1.0, V, n, s->stream));  // b - A x
+        if (x_zero) {
+            // r = b for x = 0: no operator apply (a whole sweep) needed
+            T2_HIP(hipMemcpyAsync(V, b, sizeof(double) * n,
+                                  hipMemcpyDeviceToDevice, s->stream));
+        } else {
+            T2_TRY(transport_apply(s, x, V));
+            T2_HIP(t2::launch_axpby(1.0, b, -1.0, V, n, s->stream));  // b - Ax
+        }
         double beta2 = 0.0;
         T2_TRY(dot(s, V, V, n, &beta2));
         const double beta = std::sqrt(beta2);
@@ -795,22 +802,12 @@ int gmres(t2_sweeper *s, const double *b, double *x, int64_t m, double rtol,
         T2_HIP(t2::launch_multi_axpy(V, n, int(j), s->d_scal, x, n,
                                      s->stream));
         T2_HIP(hipStreamSynchronize(s->stream));
+        x_zero = false;
     }
     if (n_iter)
         *n_iter = it;
     if (rel_res)
         *rel_res = rnorm / (bnorm > 0 ? bnorm : 1.0);
-    return T2_OK;
-}
-
-int max_abs(t2_sweeper *s, const double *x, double *out)
-{
-    std::vector<double> h(s->n_phi());
-    T2_TRY(fetch(s, h.data(), x, s->n_phi()));
-    double mx = 0.0;
-    for (double v : h)
-        mx = std::max(mx, std::fabs(v));
-    *out = mx;
     return T2_OK;
 }
 
@@ -1150,6 +1147,7 @@ static int solve_refined_body(t2_sweeper *s, int64_t iter_max, double rtol_sweep
     if (!s->d_src64)
         T2_TRY(dalloc_t(s, &s->d_src64, sizeof(double) * n));
     t2_refine_info st{};
+    const int64_t sweeps0 = s->n_sweeps;
     T2_TRY(zero_psi(s));
 
     // 1. FP32-sweep GMRES on phi, then psi32 and phi1 = D psi32
@@ -1189,18 +1187,19 @@ static int solve_refined_body(t2_sweeper *s, int64_t iter_max, double rtol_sweep
     T2_TRY(psi_to_phi(s, b));
     T2_TRY(gmres(s, b, y, restart, rtol_correction, iter_max, V,
                  &st.iters_correction, nullptr, &st.residual_correction));
-    T2_TRY(total_source(s, nullptr, y));
-    T2_TRY(run_sweep(s, nullptr, s->d_res));
-    T2_TRY(psi_to_phi(s, b));                     // b = D delta
+    // y is D delta to the correction's tolerance: phi = phi1 + y
     double corr = 0.0, base = 0.0;
-    T2_TRY(max_abs(s, b, &corr));
-    T2_HIP(t2::launch_axpby(1.0, phi_1, 1.0, b, n, s->stream));
-    T2_HIP(hipMemcpyAsync(s->d_phi, b, sizeof(double) * n,
+    T2_TRY(dot(s, y, y, n, &corr));
+    T2_HIP(t2::launch_axpby(1.0, phi_1, 1.0, y, n, s->stream));
+    T2_HIP(hipMemcpyAsync(s->d_phi, y, sizeof(double) * n,
                           hipMemcpyDeviceToDevice, s->stream));
-    T2_TRY(max_abs(s, s->d_phi, &base));
-    st.correction_rel = base > 0 ? corr / base : 0.0;
-    st.sweeps = st.iters_sweep + st.iters_correction + 4 +
-                (st.iters_sweep > 0) + (st.iters_correction > 0);
+    T2_TRY(dot(s, s->d_phi, s->d_phi, n, &base));
+    st.correction_rel = base > 0 ? std::sqrt(corr / base) : 0.0;
+    // psi consistent with the final phi (FP32 storage)
+    T2_TRY(total_source(s, s->d_qiso, s->d_phi));
+    T2_TRY(run_sweep(s, s->d_qang));
+    T2_HIP(hipStreamSynchronize(s->stream));
+    st.sweeps = s->n_sweeps - sweeps0;
     if (info)
         *info = st;
     return T2_OK;

@@ -18,6 +18,8 @@ constexpr double kInv12 = 1.0 / 12.0;
 // s_waitcnt operand (gfx9 encoding) waiting for vmcnt = 0 only: expcnt and
 // lgkmcnt fields at their maxima.
 constexpr int kWaitVmcnt0 = 0x0F70;
+// Accelerator complex dies (XCDs) of an MI355X, each with its own L2.
+constexpr int kXcds = 8;
 constexpr double kInv20 = 1.0 / 20.0;
 
 // Omega . n with each product and sum rounded separately: the host level
@@ -295,7 +297,13 @@ __global__ void __launch_bounds__(kMaxBlock)
 sweep_step_kernel(const SweepArgs<T> p)
 {
     constexpr int LPI = G / GPL;
-    const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    // Blocks go to the XCDs round robin; XCD x takes the x-th contiguous
+    // slice of the step (the grid is a multiple of kXcds), so the same
+    // angles land on the same XCD step after step.
+    const int64_t per_xcd = gridDim.x / kXcds;
+    const int64_t lb =
+        int64_t(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+    const int64_t tid = lb * blockDim.x + threadIdx.x;
     const int64_t item = tid / LPI;
     const int gl = int(tid % LPI);
     if (item >= p.n_items)
@@ -547,7 +555,8 @@ hipError_t launch_g(const SweepArgs<T> &args, int gpl, int block,
         return hipSuccess;
     const int64_t lanes_per_item = G / gpl;
     const int64_t threads = args.n_items * lanes_per_item;
-    const unsigned grid = unsigned((threads + block - 1) / block);
+    const int64_t blocks = (threads + block - 1) / block;
+    const unsigned grid = unsigned((blocks + kXcds - 1) / kXcds * kXcds);
     switch (gpl) {
     case 1:
         hipLaunchKernelGGL((sweep_step_kernel<T, G, 1>), dim3(grid),
