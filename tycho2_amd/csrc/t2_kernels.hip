@@ -746,6 +746,9 @@ __global__ void psi_to_phi_kernel(const T *__restrict__ psi,
 #pragma unroll
         for (int e = 0; e < V; e++)
             acc[e] = 0.0;
+        // angles added in ascending order; unrolled so that several angle
+        // planes' loads are in flight at once
+#pragma unroll 8
         for (int64_t a = 0; a < na; a++) {
             const Vec16<T> x = src[size_t(a) * nv + i];
             const double w = wt[a];
