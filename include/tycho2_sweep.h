@@ -252,6 +252,17 @@ int t2_batch_sweep_step(t2_batch *b);
 int t2_batch_sweep_only(t2_batch *b);
 /* longest member schedule (steps) and total (angle, cell) items */
 int t2_batch_info(const t2_batch *b, int64_t *n_steps, int64_t *n_items);
+/* t2_solve_krylov / t2_solve_source_iteration of every member, advanced in
+ * lock step: each round, every unfinished member's next operator apply (or
+ * SI step) shares one batched sweep; finished members drop out of the sweeps.
+ * Each member's phi and psi equal its own solve's bit for bit.  Per member
+ * (arrays of the member count, may be NULL): iterations, and the wall time
+ * from the call to the member's converged phi (seconds). */
+int t2_batch_solve_krylov(t2_batch *b, int64_t iter_max, double rtol,
+                          int64_t restart, int64_t *n_iter, double *seconds);
+int t2_batch_solve_source_iteration(t2_batch *b, int64_t iter_max,
+                                    double err_max, int64_t *n_iter,
+                                    double *seconds);
 int t2_batch_synchronize(t2_batch *b);
 int t2_batch_get_stream(const t2_batch *b, void **stream);
 

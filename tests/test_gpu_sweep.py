@@ -307,6 +307,40 @@ def test_batch_sweep_bitwise(precision, engine, groups):
         s.close()
 
 
+@pytest.mark.parametrize("precision,engine,method", [
+    (FP64, 0, "krylov"), (FP64, 1, "krylov"), (FP32, 1, "krylov"),
+    (FP64, 1, "si"), (FP64, 0, "si")])
+def test_batch_solves_bitwise(precision, engine, method):
+    """Lock-step batched solves give every member its own solve's phi and
+    psi bit for bit, and its own iteration count."""
+    from tycho2_amd.sweeper import SweepBatch
+    print(f"batch solve precision={precision} engine={engine} {method}",
+          flush=True)
+    groups = (4, 16, 1)
+    own = batch_members(precision, groups)
+    mem = batch_members(precision, groups)
+    counts = []
+    for s in own:
+        if method == "krylov":
+            counts.append(len(s.solve_krylov(100, 1e-10, 10)))
+        else:
+            counts.append(len(s.solve_source_iteration(300, 1e-10)))
+    with SweepBatch(mem) as b:
+        b.set_engine(engine, 1 << 20)
+        if method == "krylov":
+            its, sec = b.solve_krylov(100, 1e-10, 10)
+        else:
+            its, sec = b.solve_source_iteration(300, 1e-10)
+    print(f"  iterations {list(its)} seconds {[round(x, 3) for x in sec]}")
+    assert list(its) == counts
+    assert all(x > 0 for x in sec)
+    for s, m in zip(own, mem):
+        assert np.array_equal(s.phi(), m.phi())
+        assert np.array_equal(s.psi_angle(2), m.psi_angle(2))
+    for s in own + mem:
+        s.close()
+
+
 def test_batch_rejects_changed_member():
     from tycho2_amd.sweeper import SweepBatch, T2Error
     print("batch stale member", flush=True)

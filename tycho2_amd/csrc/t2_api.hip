@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -714,81 +715,54 @@ int transport_apply(t2_sweeper *s, const double *v, double *y)
 // Givens rotations, all in FP64 on the device; stops at rtol * ||b|| or after
 // iter_max matrix applications.  The reference delegates this to PETSc's
 // KSPGMRES (src/KrylovSolver.cc:79-169).
-int gmres(t2_sweeper *s, const double *b, double *x, int64_t m, double rtol,
-          int64_t iter_max, double *V, int64_t *n_iter, double *hist,
-          double *rel_res)
-{
-    const int64_t n = s->n_phi();
-    double bb = 0.0;
-    T2_TRY(dot(s, b, b, n, &bb));
-    const double bnorm = std::sqrt(bb), target = rtol * bnorm;
-    T2_HIP(t2::launch_fill(x, 0.0, n, s->stream));
-    std::vector<double> H(size_t(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1),
-        y(m), h(m + 1), h2(m + 1);
-    int64_t it = 0;
-    double rnorm = bnorm;
+//
+// Written as a state machine so that one driver can advance many solves
+// with one batched sweep per step: next() runs until the iteration needs
+// y = A v (returns with *need = true and req_in / req_out set; the caller
+// applies the operator and calls next() again) or until it is finished.
+struct Gmres {
+    enum State { kCycle, kResidual, kStep, kArnoldi, kDone };
+    t2_sweeper *s = nullptr;
+    const double *b = nullptr;
+    double *x = nullptr, *V = nullptr, *hist = nullptr;
+    int64_t n = 0, m = 0, iter_max = 0, it = 0, j = 0;
+    double bnorm = 0, target = 0, rnorm = 0;
     bool x_zero = true;
-    while (it < iter_max && rnorm > target) {
-        if (x_zero) {
-            // r = b for x = 0: no operator apply (a whole sweep) needed
-            T2_HIP(hipMemcpyAsync(V, b, sizeof(double) * n,
-                                  hipMemcpyDeviceToDevice, s->stream));
-        } else {
-            T2_TRY(transport_apply(s, x, V));
-            T2_HIP(t2::launch_axpby(1.0, b, -1.0, V, n, s->stream));  // b - Ax
-        }
-        double beta2 = 0.0;
-        T2_TRY(dot(s, V, V, n, &beta2));
-        const double beta = std::sqrt(beta2);
-        rnorm = beta;
-        if (beta <= target || beta == 0.0)
-            break;
-        T2_HIP(t2::launch_scale(V, 1.0 / beta, n, s->stream));
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0] = beta;
-        int64_t j = 0;
-        for (; j < m && it < iter_max; j++) {
-            double *w = V + n * (j + 1);
-            T2_TRY(transport_apply(s, V + n * j, w));
-            for (int pass = 0; pass < 2; pass++) {
-                T2_HIP(t2::launch_multi_dot(V, n, int(j + 1), w, n, s->d_red,
-                                            s->d_scal, s->stream));
-                T2_HIP(t2::launch_multi_axpy(V, n, int(j + 1), s->d_scal, w,
-                                             n, s->stream));
-                T2_TRY(fetch(s, pass ? h2.data() : h.data(), s->d_scal,
-                             j + 1));
-            }
-            for (int64_t i = 0; i <= j; i++)
-                H[i * m + j] = h[i] + h2[i];
-            double ww = 0.0;
-            T2_TRY(dot(s, w, w, n, &ww));
-            const double hn = std::sqrt(ww);
-            H[(j + 1) * m + j] = hn;
-            if (hn != 0.0)
-                T2_HIP(t2::launch_scale(w, 1.0 / hn, n, s->stream));
-            for (int64_t i = 0; i < j; i++) {
-                const double t = cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
-                H[(i + 1) * m + j] =
-                    -sn[i] * H[i * m + j] + cs[i] * H[(i + 1) * m + j];
-                H[i * m + j] = t;
-            }
-            const double hjj = H[j * m + j], hj1 = H[(j + 1) * m + j];
-            const double r = std::sqrt(hjj * hjj + hj1 * hj1);
-            cs[j] = hjj / r;
-            sn[j] = hj1 / r;
-            H[j * m + j] = r;
-            H[(j + 1) * m + j] = 0.0;
-            g[j + 1] = -sn[j] * g[j];
-            g[j] = cs[j] * g[j];
-            rnorm = std::fabs(g[j + 1]);
-            if (hist)
-                hist[it] = rnorm / (bnorm > 0 ? bnorm : 1.0);
-            it++;
-            if (rnorm <= target) {
-                j++;
-                break;
-            }
-        }
+    State state = kCycle;
+    std::vector<double> H, cs, sn, g, y, h, h2;
+    const double *req_in = nullptr;
+    double *req_out = nullptr;
+
+    int init(t2_sweeper *s_, const double *b_, double *x_, int64_t m_,
+             double rtol, int64_t iter_max_, double *V_, double *hist_)
+    {
+        s = s_;
+        b = b_;
+        x = x_;
+        m = m_;
+        iter_max = iter_max_;
+        V = V_;
+        hist = hist_;
+        n = s->n_phi();
+        double bb = 0.0;
+        T2_TRY(dot(s, b, b, n, &bb));
+        bnorm = std::sqrt(bb);
+        target = rtol * bnorm;
+        rnorm = bnorm;
+        T2_HIP(t2::launch_fill(x, 0.0, n, s->stream));
+        H.assign(size_t(m + 1) * m, 0.0);
+        cs.assign(m, 0.0);
+        sn.assign(m, 0.0);
+        g.assign(m + 1, 0.0);
+        y.assign(m, 0.0);
+        h.assign(m + 1, 0.0);
+        h2.assign(m + 1, 0.0);
+        state = kCycle;
+        return T2_OK;
+    }
+
+    int end_cycle()
+    {
         for (int64_t i = j - 1; i >= 0; i--) {
             double acc = g[i];
             for (int64_t k = i + 1; k < j; k++)
@@ -803,11 +777,131 @@ int gmres(t2_sweeper *s, const double *b, double *x, int64_t m, double rtol,
                                      s->stream));
         T2_HIP(hipStreamSynchronize(s->stream));
         x_zero = false;
+        state = kCycle;
+        return T2_OK;
+    }
+
+    int next(bool *need)
+    {
+        *need = false;
+        for (;;) {
+            switch (state) {
+            case kCycle:
+                if (!(it < iter_max && rnorm > target)) {
+                    state = kDone;
+                    break;
+                }
+                state = kResidual;
+                if (x_zero) {
+                    // r = b for x = 0: no operator apply (a whole sweep)
+                    T2_HIP(hipMemcpyAsync(V, b, sizeof(double) * n,
+                                          hipMemcpyDeviceToDevice,
+                                          s->stream));
+                    break;
+                }
+                req_in = x;
+                req_out = V;
+                *need = true;
+                return T2_OK;
+            case kResidual: {
+                if (!x_zero)  // V = b - A x
+                    T2_HIP(t2::launch_axpby(1.0, b, -1.0, V, n, s->stream));
+                double beta2 = 0.0;
+                T2_TRY(dot(s, V, V, n, &beta2));
+                const double beta = std::sqrt(beta2);
+                rnorm = beta;
+                if (beta <= target || beta == 0.0) {
+                    state = kDone;
+                    break;
+                }
+                T2_HIP(t2::launch_scale(V, 1.0 / beta, n, s->stream));
+                std::fill(g.begin(), g.end(), 0.0);
+                g[0] = beta;
+                j = 0;
+                state = kStep;
+                break;
+            }
+            case kStep:
+                if (!(j < m && it < iter_max)) {
+                    T2_TRY(end_cycle());
+                    break;
+                }
+                req_in = V + n * j;
+                req_out = V + n * (j + 1);
+                state = kArnoldi;
+                *need = true;
+                return T2_OK;
+            case kArnoldi: {
+                double *w = V + n * (j + 1);
+                for (int pass = 0; pass < 2; pass++) {
+                    T2_HIP(t2::launch_multi_dot(V, n, int(j + 1), w, n,
+                                                s->d_red, s->d_scal,
+                                                s->stream));
+                    T2_HIP(t2::launch_multi_axpy(V, n, int(j + 1), s->d_scal,
+                                                 w, n, s->stream));
+                    T2_TRY(fetch(s, pass ? h2.data() : h.data(), s->d_scal,
+                                 j + 1));
+                }
+                for (int64_t i = 0; i <= j; i++)
+                    H[i * m + j] = h[i] + h2[i];
+                double ww = 0.0;
+                T2_TRY(dot(s, w, w, n, &ww));
+                const double hn = std::sqrt(ww);
+                H[(j + 1) * m + j] = hn;
+                if (hn != 0.0)
+                    T2_HIP(t2::launch_scale(w, 1.0 / hn, n, s->stream));
+                for (int64_t i = 0; i < j; i++) {
+                    const double t =
+                        cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
+                    H[(i + 1) * m + j] =
+                        -sn[i] * H[i * m + j] + cs[i] * H[(i + 1) * m + j];
+                    H[i * m + j] = t;
+                }
+                const double hjj = H[j * m + j], hj1 = H[(j + 1) * m + j];
+                const double r = std::sqrt(hjj * hjj + hj1 * hj1);
+                cs[j] = hjj / r;
+                sn[j] = hj1 / r;
+                H[j * m + j] = r;
+                H[(j + 1) * m + j] = 0.0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] = cs[j] * g[j];
+                rnorm = std::fabs(g[j + 1]);
+                if (hist)
+                    hist[it] = rnorm / (bnorm > 0 ? bnorm : 1.0);
+                it++;
+                j++;
+                if (rnorm <= target)
+                    T2_TRY(end_cycle());
+                else
+                    state = kStep;
+                break;
+            }
+            case kDone:
+                return T2_OK;
+            }
+        }
+    }
+
+    double rel_res() const { return rnorm / (bnorm > 0 ? bnorm : 1.0); }
+};
+
+int gmres(t2_sweeper *s, const double *b, double *x, int64_t m, double rtol,
+          int64_t iter_max, double *V, int64_t *n_iter, double *hist,
+          double *rel_res)
+{
+    Gmres gm;
+    T2_TRY(gm.init(s, b, x, m, rtol, iter_max, V, hist));
+    for (;;) {
+        bool need = false;
+        T2_TRY(gm.next(&need));
+        if (!need)
+            break;
+        T2_TRY(transport_apply(s, gm.req_in, gm.req_out));
     }
     if (n_iter)
-        *n_iter = it;
+        *n_iter = gm.it;
     if (rel_res)
-        *rel_res = rnorm / (bnorm > 0 ? bnorm : 1.0);
+        *rel_res = gm.rel_res();
     return T2_OK;
 }
 
@@ -1366,6 +1460,7 @@ namespace {
 struct LevelGroup {
     int ng = 0;
     std::vector<int> idx;                // member indices
+    int32_t *d_active = nullptr;         // active flags in group order
     std::vector<int64_t> step_off;       // merged schedule
     int64_t n_items = 0;
     void *d_geo = nullptr;
@@ -1394,6 +1489,9 @@ struct t2_batch {
     t2::BatchChunk *d_chunks = nullptr;
     unsigned long long *d_ticket = nullptr;
     int32_t *d_abort = nullptr;
+    int32_t *d_active = nullptr;         // members swept by the next sweep
+    std::vector<int32_t> active;         // host copy (kept alive for copies)
+    std::vector<int32_t> uploaded;       // flags the device holds
     hipGraphExec_t graph = nullptr;
     std::vector<std::vector<const void *>> snap;  // members' buffers
 };
@@ -1439,6 +1537,8 @@ int batch_fail_cleanup(t2_batch *b, int rc)
 void free_levels(t2_batch *b)
 {
     for (auto &g : b->groups) {
+        if (g.d_active)
+            (void)hipFree(g.d_active);
         if (g.d_geo)
             (void)hipFree(g.d_geo);
         if (g.d_topo)
@@ -1530,6 +1630,8 @@ int build_level_group(t2_batch *b, LevelGroup &g)
         desc[j].lag_buf = static_cast<const T *>(m->d_lag_buf);
     }
     (void)hipFree(d_off);
+    T2_HIP(hipMalloc(reinterpret_cast<void **>(&g.d_active),
+                     sizeof(int32_t) * g.idx.size()));
     T2_HIP(hipMalloc(&g.d_desc, sizeof(t2::ProblemDesc<T>) * desc.size()));
     T2_HIP(hipMemcpy(g.d_desc, desc.data(),
                      sizeof(t2::ProblemDesc<T>) * desc.size(),
@@ -1641,6 +1743,7 @@ int batch_enqueue(t2_batch *b)
         q.ticket = b->d_ticket;
         q.abort = b->d_abort;
         q.spin_limit = b->spin_limit > 0 ? b->spin_limit : kDefaultSpinLimit;
+        q.active = b->d_active;
         T2_HIP(t2::launch_fill(reinterpret_cast<double *>(b->d_ticket), 0.0,
                                1, b->stream));
         T2_HIP(t2::launch_queue_batch<T>(
@@ -1657,6 +1760,7 @@ int batch_enqueue(t2_batch *b)
                 continue;
             t2::SweepArgs<T> a{};
             a.problems = static_cast<const t2::ProblemDesc<T> *>(g.d_desc);
+            a.active = g.d_active;
             a.geo = static_cast<const t2::ItemGeo<T> *>(g.d_geo) +
                     g.step_off[st];
             a.topo = g.d_topo + g.step_off[st];
@@ -1700,9 +1804,29 @@ int batch_sweep(t2_batch *b)
         T2_HIP(hipStreamSynchronize(b->stream));
         T2_TRY(build_levels(b));
         batch_snapshot(b);
+        b->uploaded.clear();
     }
-    for (auto *m : b->members)
-        m->n_sweeps++;
+    // active flags (all members unless a solve masks some), uploaded when
+    // they change; the kernels of the captured sweep read them
+    for (size_t i = 0; i < b->members.size(); i++)
+        if (b->active[i])
+            b->members[i]->n_sweeps++;
+    if (b->active != b->uploaded) {
+        T2_HIP(hipMemcpyAsync(b->d_active, b->active.data(),
+                              sizeof(int32_t) * b->active.size(),
+                              hipMemcpyHostToDevice, b->stream));
+        for (auto &g : b->groups) {
+            std::vector<int32_t> ga(g.idx.size());
+            for (size_t j = 0; j < g.idx.size(); j++)
+                ga[j] = b->active[g.idx[j]];
+            T2_HIP(hipMemcpyAsync(g.d_active, ga.data(),
+                                  sizeof(int32_t) * ga.size(),
+                                  hipMemcpyHostToDevice, b->stream));
+            T2_HIP(hipStreamSynchronize(b->stream));
+        }
+        T2_HIP(hipStreamSynchronize(b->stream));
+        b->uploaded = b->active;
+    }
     if (!b->graph) {
         hipGraph_t g = nullptr;
         T2_HIP(hipStreamBeginCapture(b->stream,
@@ -1790,6 +1914,10 @@ int t2_batch_create(t2_batch **out, t2_sweeper *const *members, int count)
     if (hipEventCreateWithFlags(&b->order, hipEventDisableTiming) !=
         hipSuccess)
         return batch_fail_cleanup(b, fail(T2_ERR_HIP, "event creation"));
+    b->active.assign(count, 1);
+    if (hipMalloc(reinterpret_cast<void **>(&b->d_active),
+                  sizeof(int32_t) * count) != hipSuccess)
+        return batch_fail_cleanup(b, fail(T2_ERR_NOMEM, "batch flags"));
     for (auto *m : b->members)
         if (hipStreamSynchronize(m->stream) != hipSuccess)
             return batch_fail_cleanup(b, fail(T2_ERR_HIP, "member sync"));
@@ -1812,6 +1940,8 @@ int t2_batch_destroy(t2_batch *b)
         (void)hipFree(b->d_ticket);
     if (b->d_abort)
         (void)hipFree(b->d_abort);
+    if (b->d_active)
+        (void)hipFree(b->d_active);
     if (b->order)
         (void)hipEventDestroy(b->order);
     if (b->stream)
@@ -1886,6 +2016,160 @@ int t2_batch_sweep_step(t2_batch *b)
                 m->n_phi(), m->d_phi, b->stream));
     }
     return members_after_batch(b);
+}
+
+// One batched sweep of the members flagged in act (their total sources set
+// on their own streams beforehand).
+static int batch_sweep_some(t2_batch *b, const std::vector<int32_t> &act)
+{
+    b->active = act;
+    T2_TRY(batch_after_members(b));
+    int rc = batch_sweep(b);
+    std::fill(b->active.begin(), b->active.end(), 1);
+    T2_TRY(rc);
+    return members_after_batch(b);
+}
+
+static double seconds_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                         t0)
+        .count();
+}
+
+int t2_batch_solve_krylov(t2_batch *b, int64_t iter_max, double rtol,
+                          int64_t restart, int64_t *n_iter, double *seconds)
+{
+    t2::clear_error();
+    T2_TRY(batch_check(b));
+    if (restart < 1 || restart > 60 || iter_max < 0)
+        return fail(T2_ERR_ARG, "restart must be in [1, 60]");
+    T2_HIP(hipSetDevice(b->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t nm = b->members.size();
+    if (n_iter)
+        std::fill(n_iter, n_iter + nm, 0);
+    if (seconds)
+        std::fill(seconds, seconds + nm, 0.0);
+    std::vector<double *> V(nm), rhs(nm);
+    for (size_t i = 0; i < nm; i++) {
+        t2_sweeper *m = b->members[i];
+        double *work = nullptr;
+        T2_TRY(krylov_space(m, restart, 1, &V[i], &work));
+        rhs[i] = work;
+        T2_TRY(zero_psi(m));
+        T2_TRY(total_source(m, m->d_qiso, nullptr));  // b = D L^-1 q
+    }
+    std::vector<int32_t> act(nm, 1);
+    T2_TRY(batch_sweep_some(b, act));
+    std::vector<Gmres> gm(nm);
+    for (size_t i = 0; i < nm; i++) {
+        t2_sweeper *m = b->members[i];
+        T2_TRY(psi_to_phi(m, rhs[i]));
+        T2_TRY(gm[i].init(m, rhs[i], m->d_phi, restart, rtol, iter_max, V[i],
+                          nullptr));
+    }
+    // every member's GMRES advances to its next operator apply; one
+    // batched sweep serves all the applies of a round
+    std::vector<bool> done(nm, false);
+    for (;;) {
+        bool any = false;
+        for (size_t i = 0; i < nm; i++) {
+            act[i] = 0;
+            if (done[i])
+                continue;
+            bool need = false;
+            T2_TRY(gm[i].next(&need));
+            if (need) {
+                act[i] = 1;
+                any = true;
+                T2_TRY(total_source(b->members[i], nullptr, gm[i].req_in));
+            } else {
+                done[i] = true;
+                if (n_iter)
+                    n_iter[i] = gm[i].it;
+                if (seconds)
+                    seconds[i] = seconds_since(t0);
+            }
+        }
+        if (!any)
+            break;
+        T2_TRY(batch_sweep_some(b, act));
+        for (size_t i = 0; i < nm; i++) {
+            if (!act[i])
+                continue;
+            t2_sweeper *m = b->members[i];
+            T2_TRY(psi_to_phi(m, gm[i].req_out));
+            T2_HIP(t2::launch_axpby(1.0, gm[i].req_in, -1.0, gm[i].req_out,
+                                    m->n_phi(), m->stream));
+        }
+    }
+    // psi = L^-1 (q + M S phi) for every member
+    for (auto *m : b->members)
+        T2_TRY(total_source(m, m->d_qiso, m->d_phi));
+    std::fill(act.begin(), act.end(), 1);
+    T2_TRY(batch_sweep_some(b, act));
+    for (auto *m : b->members)
+        T2_HIP(hipStreamSynchronize(m->stream));
+    return batch_check_abort(b);
+}
+
+int t2_batch_solve_source_iteration(t2_batch *b, int64_t iter_max,
+                                    double err_max, int64_t *n_iter,
+                                    double *seconds)
+{
+    t2::clear_error();
+    T2_TRY(batch_check(b));
+    if (iter_max < 0)
+        return fail(T2_ERR_ARG, "iter_max < 0");
+    T2_HIP(hipSetDevice(b->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t nm = b->members.size();
+    if (n_iter)
+        std::fill(n_iter, n_iter + nm, 0);
+    if (seconds)
+        std::fill(seconds, seconds + nm, 0.0);
+    std::vector<int64_t> it(nm, 0);
+    std::vector<double> err(nm, 1.0);
+    for (auto *m : b->members) {
+        T2_TRY(zero_psi(m));
+        T2_HIP(t2::launch_fill(m->d_phi, 0.0, m->n_phi(), m->stream));
+    }
+    std::vector<int32_t> act(nm, 0);
+    for (;;) {
+        bool any = false;
+        for (size_t i = 0; i < nm; i++) {
+            t2_sweeper *m = b->members[i];
+            act[i] = it[i] < iter_max && err[i] > err_max;
+            if (act[i]) {
+                any = true;
+                T2_TRY(total_source(m, m->d_qiso, m->d_phi));
+            }
+        }
+        if (!any)
+            break;
+        T2_TRY(batch_sweep_some(b, act));
+        for (size_t i = 0; i < nm; i++) {
+            if (!act[i])
+                continue;
+            t2_sweeper *m = b->members[i];
+            std::swap(m->d_phi, m->d_phi_old);
+            T2_TRY(psi_to_phi(m, m->d_phi));
+            T2_HIP(t2::launch_l1_change(m->d_phi, m->d_phi_old, m->n_phi(),
+                                        m->d_red, m->d_scal, m->stream));
+            double sums[2];
+            T2_TRY(fetch(m, sums, m->d_scal, 2));
+            err[i] = sums[0] / sums[1];
+            it[i]++;
+            if (!(it[i] < iter_max && err[i] > err_max)) {
+                if (n_iter)
+                    n_iter[i] = it[i];
+                if (seconds)
+                    seconds[i] = seconds_since(t0);
+            }
+        }
+    }
+    return batch_check_abort(b);
 }
 
 int t2_batch_info(const t2_batch *b, int64_t *n_steps, int64_t *n_items)

@@ -309,6 +309,8 @@ sweep_step_kernel(const SweepArgs<T> p)
     if (item >= p.n_items)
         return;
     const ItemTopo t = p.topo[item];
+    if (p.active && !p.active[t.problem])
+        return;
     solve_item<T, G, GPL>(p, item, gl, t);
 }
 
@@ -453,6 +455,8 @@ queue_batch_kernel(const BatchProblem<T> *problems, const BatchChunk *chunks,
         if (c >= q.n_chunks)
             break;
         const BatchChunk ch = chunks[c];
+        if (q.active && !q.active[ch.problem])
+            continue;  // whole chunk skipped: its counters stay armed
         const BatchProblem<T> &bp = problems[ch.problem];
         const int64_t lo = ch.lo, hi = ch.lo + ch.n;
         switch (bp.groups) {

@@ -71,8 +71,10 @@ struct SweepArgs {
     T *psi;                    // [na][nc][4][G]
     const T *lag_buf;          // [n_lag][4][G] upwind values held pre-sweep
     // batched launch: per-problem pointers replace nc, sigma, src_iso, psi
-    // and lag_buf (src_ang and src_ang_t must be null)
+    // and lag_buf (src_ang and src_ang_t must be null); items of problems
+    // whose active[] flag is 0 are skipped (active may be null: all)
     const ProblemDesc<T> *problems;
+    const int32_t *active;
 };
 
 // Defect of the coupled transport equations for a stored angular flux,
@@ -140,6 +142,7 @@ struct QueueState {
     unsigned long long *ticket;    // next chunk to hand out (0 at the start)
     int32_t *abort;                // set when a wait exceeds spin_limit
     int64_t spin_limit;            // polls per wait before giving up
+    const int32_t *active;         // batched launch: problems to sweep
 };
 
 template <typename T>

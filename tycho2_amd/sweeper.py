@@ -85,6 +85,11 @@ SIGNATURES = {
     "t2_batch_sweep_step": (ctypes.c_int, [ctypes.c_void_p]),
     "t2_batch_sweep_only": (ctypes.c_int, [ctypes.c_void_p]),
     "t2_batch_info": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip]),
+    "t2_batch_solve_krylov": (ctypes.c_int, [
+        ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int64,
+        _ip, _dp]),
+    "t2_batch_solve_source_iteration": (ctypes.c_int, [
+        ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, _ip, _dp]),
     "t2_batch_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "t2_batch_get_stream": (ctypes.c_int, [ctypes.c_void_p,
                                            ctypes.POINTER(ctypes.c_void_p)]),
@@ -413,6 +418,28 @@ class SweepBatch:
         s = ctypes.c_void_p()
         _check(lib().t2_batch_get_stream(self.handle, ctypes.byref(s)))
         return s.value or 0
+
+    def solve_krylov(self, iter_max: int = 200, rtol: float = 1e-10,
+                     restart: int = 20):
+        """Every member's GMRES solve in lock step; returns (iterations,
+        seconds to converged phi) per member."""
+        n = len(self.members)
+        its = np.zeros(n, np.int64)
+        sec = np.zeros(n)
+        _check(lib().t2_batch_solve_krylov(self.handle, int(iter_max),
+                                           float(rtol), int(restart),
+                                           _ptr(its, _ip), _ptr(sec)))
+        return its, sec
+
+    def solve_source_iteration(self, iter_max: int = 1000,
+                               err_max: float = 1e-10):
+        n = len(self.members)
+        its = np.zeros(n, np.int64)
+        sec = np.zeros(n)
+        _check(lib().t2_batch_solve_source_iteration(
+            self.handle, int(iter_max), float(err_max), _ptr(its, _ip),
+            _ptr(sec)))
+        return its, sec
 
     def info(self) -> dict:
         n_steps, n_items = ctypes.c_int64(), ctypes.c_int64()
