@@ -163,6 +163,104 @@ def counter_run(p, precision, sweeps):
     log(f"counter run: {sweeps} sweeps, calibration copy {nbytes} B")
 
 
+def batch_bytes(p) -> int:
+    """Device bytes of one FP64 batch member: psi, item records, a merged
+    copy of the records (level form), ready-queue tables, Krylov space."""
+    items = p.nc * p.na
+    return items * (32 * p.ng + 80 + 80 + 20) + p.nc * 4 * p.ng * 8 * 30
+
+
+def batch_leg(args, ev):
+    """The 48-problem workload: waves of problems that fit the device, each
+    swept as one batch by both engines and solved by lock-step GMRES."""
+    import numpy as np
+    from tycho2_amd import problems
+    from tycho2_amd.sweeper import (ENGINE_LEVELS, ENGINE_QUEUE, FP64,
+                                    SweepBatch, SweeperHIP)
+    from tycho2_amd.timing import time_region
+    t0 = time.time()
+    ps = problems.batch(count=args.batch_count)
+    log(f"batch: {len(ps)} problems generated in {time.time() - t0:.0f}s")
+    budget = args.batch_gb * 1e9
+    waves, cur, used = [], [], 0
+    for p in sorted(ps, key=batch_bytes, reverse=True):
+        if cur and used + batch_bytes(p) > budget:
+            waves.append(cur)
+            cur, used = [], 0
+        cur.append(p)
+        used += batch_bytes(p)
+    waves.append(cur)
+    out = dict(problems=len(ps), waves=[], per_problem=[])
+    tot = dict(levels=0.0, queue=0.0, best=0.0, solves=0, solve_s=0.0)
+    for w, wave in enumerate(waves):
+        t1 = time.time()
+        hs = []
+        for p in wave:
+            hs.append(SweeperHIP.from_problem(p, FP64))
+            log(f"wave {w}: handle {len(hs)}/{len(wave)} ({p.nc} cells, "
+                f"{p.na} angles, {p.ng} groups)")
+        log(f"wave {w}: {len(wave)} problems, handles in "
+            f"{time.time() - t1:.0f}s")
+        solves = sum(p.solves_per_sweep for p in wave)
+        rec = dict(problems=len(wave), solves_per_sweep=solves)
+        with SweepBatch(hs) as b:
+            timings = {}
+            for name, engine, rounds in (("levels", ENGINE_LEVELS, 0),
+                                         ("queue_r2", ENGINE_QUEUE, 2),
+                                         ("queue_r4", ENGINE_QUEUE, 4),
+                                         ("queue_r8", ENGINE_QUEUE, 8)):
+                b.set_engine(engine, 0, rounds)
+                b.sweep_only()
+                b.synchronize()
+                ms = time_region(ev, b, b.sweep_only, args.steps) / args.steps
+                timings[name] = ms
+                log(f"wave {w}: {name} {ms:.2f} ms/sweep, "
+                    f"{solves / ms / 1e6:.1f} G solves/s")
+            best_q = min((k for k in timings if k.startswith("queue")),
+                         key=timings.get)
+            rec["ms_per_sweep"] = {k: round(v, 3) for k, v in timings.items()}
+            rec["engine"] = ("queue" if timings[best_q] < timings["levels"]
+                             else "levels")
+            tot["levels"] += timings["levels"]
+            tot["queue"] += timings[best_q]
+            tot["best"] += min(timings["levels"], timings[best_q])
+            tot["solves"] += solves
+            b.set_engine(ENGINE_QUEUE if timings[best_q] < timings["levels"]
+                         else ENGINE_LEVELS, 0,
+                         int(best_q[len("queue_r"):]))
+            t2 = time.time()
+            its, sec = b.solve_krylov(300, 1e-10, 20)
+            rec["solve_s"] = round(time.time() - t2, 3)
+            tot["solve_s"] += rec["solve_s"]
+            log(f"wave {w}: batched GMRES(20) to 1e-10 in {rec['solve_s']}s, "
+                f"iterations {int(its.min())}-{int(its.max())}")
+        for p, h, k, t in zip(wave, hs, its, sec):
+            out["per_problem"].append(dict(
+                name=p.name, cells=p.nc, angles=p.na, groups=p.ng,
+                gmres_iters=int(k), time_to_solution_s=round(float(t), 3)))
+            h.close()
+        out["waves"].append(rec)
+    tts = np.array([q["time_to_solution_s"] for q in out["per_problem"]])
+    res = {"metric": "DG cell-angle-group solves/s, 48 independent problems "
+                     "batched on one GPU (FP64; each wave on its faster "
+                     "engine)",
+           "value": round(tot["solves"] / (tot["best"] * 1e-3)),
+           "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{len(ps)} seeded Kuhn cubes, n in "
+                                  "[10, 45], S4/6/8/12, 1/4/16/32 groups",
+                      "waves": len(waves)},
+           "levels_solves_per_s": round(tot["solves"] /
+                                        (tot["levels"] * 1e-3)),
+           "queue_solves_per_s": round(tot["solves"] / (tot["queue"] * 1e-3)),
+           "solve_wall_s": round(tot["solve_s"], 3),
+           "time_to_solution_s": {"min": float(tts.min()),
+                                  "median": float(np.median(tts)),
+                                  "max": float(tts.max())},
+           "detail": out}
+    print(json.dumps(res), flush=True)
+
+
 def traffic_from_profiles(workload, precision):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
@@ -182,7 +280,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="precision",
-                    choices=["plumbing", "plain", "precision", "scattering"])
+                    choices=["plumbing", "plain", "precision", "scattering",
+                             "batch"])
+    ap.add_argument("--batch-count", type=int, default=48)
+    ap.add_argument("--batch-gb", type=float, default=200.0,
+                    help="device memory per wave of the batch workload")
     ap.add_argument("--cpu-angles", type=int, default=0,
                     help="angles in the CPU-baseline sample (0: 2 x cores)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -201,6 +303,9 @@ def main() -> int:
     from tycho2_amd import problems
     from tycho2_amd.sweeper import FP32, FP64
     from tycho2_amd.timing import HipEvents
+    if args.config == "batch":
+        batch_leg(args, HipEvents())
+        return 0
     t0 = time.time()
     p = getattr(problems, args.config)()
     n = round((p.nc / 6) ** (1.0 / 3.0))

@@ -356,6 +356,7 @@ __device__ __forceinline__ void queue_chunk(const SweepArgs<T> &p,
     for (int64_t it = first; ok && it < w_hi; it += IPR) {
         const int32_t *mine = pending + (it - first_item);
         int64_t spins = 0;
+        int nap = 1;
         while (__hip_atomic_load(mine, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT) > 0) {
             if (++spins > q.spin_limit) {
@@ -364,13 +365,18 @@ __device__ __forceinline__ void queue_chunk(const SweepArgs<T> &p,
                 ok = false;
                 break;
             }
-            if ((spins & 63) == 0 &&
+            if ((spins & 15) == 0 &&
                 __hip_atomic_load(q.abort, __ATOMIC_RELAXED,
                                   __HIP_MEMORY_SCOPE_AGENT) != 0) {
                 ok = false;
                 break;
             }
-            __builtin_amdgcn_s_sleep(1);
+            // back off (up to ~1 us between polls): thousands of waiting
+            // wavefronts polling at full rate would load the memory system
+            // the solving ones need
+            for (int k = 0; k < nap; k++)
+                __builtin_amdgcn_s_sleep(2);
+            nap = nap < 16 ? 2 * nap : 16;
         }
     }
     // no upwind load may move above the waits (compiler ordering; the loads
