@@ -51,9 +51,15 @@ extern "C" {
 #define T2_FP64 0
 #define T2_FP32 1
 
-/* cell solvers, deck key GaussElim (src/Main.cc:167-176) */
-#define T2_GAUSS_NOPIVOT 1
+/* cell solvers, deck key GaussElim (src/Main.cc:167-176, values as
+ * src/Global.hh:99-105): elimination with partial pivoting, without
+ * pivoting (the fastest; every engine), Cramer's rule (both Cramer options
+ * select one adjugate solve).  Solvers other than NoPivot run on the level
+ * engine with one group per lane. */
 #define T2_GAUSS_ORIGINAL 0
+#define T2_GAUSS_NOPIVOT 1
+#define T2_GAUSS_CRAMER_GLU 2
+#define T2_GAUSS_CRAMER_INTEL 3
 
 typedef struct t2_sweeper t2_sweeper;
 

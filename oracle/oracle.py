@@ -18,6 +18,8 @@ _LIB = None
 
 NOPIVOT = 1
 ORIGINAL = 0
+CRAMER_GLU = 2
+CRAMER_INTEL = 3
 
 _d = ctypes.POINTER(ctypes.c_double)
 _i = ctypes.POINTER(ctypes.c_int64)

@@ -261,6 +261,55 @@ static void solve_pivot(double m[4][4], double b[4])
     }
 }
 
+/* Cramer's rule, x = adj(A) b / det(A), for the reference's CramerGlu and
+ * CramerIntel options (src/Transport.cc:414-673), which form the adjugate
+ * by two different cofactor expansions.  Here the adjugate comes from the
+ * Laplace expansion over complementary 2x2 minors of rows {0,1} and {2,3}
+ * (D. Eberly, "The Laplace Expansion Theorem: Computing the Determinants and
+ * Inverses of Matrices", Geometric Tools, 2008); all three agree up to
+ * rounding. */
+static void solve_cramer(double a[4][4], double b[4])
+{
+    const double s0 = a[0][0] * a[1][1] - a[1][0] * a[0][1];
+    const double s1 = a[0][0] * a[1][2] - a[1][0] * a[0][2];
+    const double s2 = a[0][0] * a[1][3] - a[1][0] * a[0][3];
+    const double s3 = a[0][1] * a[1][2] - a[1][1] * a[0][2];
+    const double s4 = a[0][1] * a[1][3] - a[1][1] * a[0][3];
+    const double s5 = a[0][2] * a[1][3] - a[1][2] * a[0][3];
+    const double c5 = a[2][2] * a[3][3] - a[3][2] * a[2][3];
+    const double c4 = a[2][1] * a[3][3] - a[3][1] * a[2][3];
+    const double c3 = a[2][1] * a[3][2] - a[3][1] * a[2][2];
+    const double c2 = a[2][0] * a[3][3] - a[3][0] * a[2][3];
+    const double c1 = a[2][0] * a[3][2] - a[3][0] * a[2][2];
+    const double c0 = a[2][0] * a[3][1] - a[3][0] * a[2][1];
+    const double det = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 +
+                       s5 * c0;
+    double adj[4][4];
+    adj[0][0] = a[1][1] * c5 - a[1][2] * c4 + a[1][3] * c3;
+    adj[0][1] = -a[0][1] * c5 + a[0][2] * c4 - a[0][3] * c3;
+    adj[0][2] = a[3][1] * s5 - a[3][2] * s4 + a[3][3] * s3;
+    adj[0][3] = -a[2][1] * s5 + a[2][2] * s4 - a[2][3] * s3;
+    adj[1][0] = -a[1][0] * c5 + a[1][2] * c2 - a[1][3] * c1;
+    adj[1][1] = a[0][0] * c5 - a[0][2] * c2 + a[0][3] * c1;
+    adj[1][2] = -a[3][0] * s5 + a[3][2] * s2 - a[3][3] * s1;
+    adj[1][3] = a[2][0] * s5 - a[2][2] * s2 + a[2][3] * s1;
+    adj[2][0] = a[1][0] * c4 - a[1][1] * c2 + a[1][3] * c0;
+    adj[2][1] = -a[0][0] * c4 + a[0][1] * c2 - a[0][3] * c0;
+    adj[2][2] = a[3][0] * s4 - a[3][1] * s2 + a[3][3] * s0;
+    adj[2][3] = -a[2][0] * s4 + a[2][1] * s2 - a[2][3] * s0;
+    adj[3][0] = -a[1][0] * c3 + a[1][1] * c1 - a[1][2] * c0;
+    adj[3][1] = a[0][0] * c3 - a[0][1] * c1 + a[0][2] * c0;
+    adj[3][2] = -a[3][0] * s3 + a[3][1] * s1 - a[3][2] * s0;
+    adj[3][3] = a[2][0] * s3 - a[2][1] * s1 + a[2][2] * s0;
+    const double inv = 1.0 / det;
+    double x[4];
+    for (int i = 0; i < 4; i++)
+        x[i] = (adj[i][0] * b[0] + adj[i][1] * b[1] + adj[i][2] * b[2] +
+                adj[i][3] * b[3]) * inv;
+    for (int i = 0; i < 4; i++)
+        b[i] = x[i];
+}
+
 /* Linear-DG system of one tet (Transport.cc:54-271, 687-733).  With
  * a_k = abar_k / 12 and s = sigma_t V / 20:
  *   M[i][j] = a_i + (1 + d_ij) s + sum_{k: abar_k > 0, k != i,j} (1 + d_ij) a_k
@@ -314,6 +363,8 @@ void or_cell_solve(int method, double volume, const double abar[4],
     cell_system(volume, abar, sigma_t, q, upwind, m, b);
     if (method == OR_GAUSS_ORIGINAL)
         solve_pivot(m, b);
+    else if (method == OR_GAUSS_CRAMER_GLU || method == OR_GAUSS_CRAMER_INTEL)
+        solve_cramer(m, b);
     else
         solve_nopivot(m, b);
     for (int i = 0; i < 4; i++)

@@ -27,7 +27,14 @@
 extern "C" {
 #endif
 
-enum { OR_GAUSS_ORIGINAL = 0, OR_GAUSS_NOPIVOT = 1 };
+/* cell solvers, the reference's GaussElim deck values (src/Global.hh:99-105);
+ * both Cramer values select one adjugate solve (see or_cell_solve) */
+enum {
+    OR_GAUSS_ORIGINAL = 0,
+    OR_GAUSS_NOPIVOT = 1,
+    OR_GAUSS_CRAMER_GLU = 2,
+    OR_GAUSS_CRAMER_INTEL = 3
+};
 
 typedef struct {
     int64_t nc;

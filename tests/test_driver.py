@@ -50,15 +50,23 @@ def test_driver_usage_and_errors(tmp_path):
     assert r.returncode == 1 and "snOrder" in r.stderr
     r = run(GOLD_DECK.format(out=out, extra="SweepEngine Fast"), tmp_path)
     assert r.returncode == 1 and "SweepEngine" in r.stderr
+    r = run(GOLD_DECK.format(out=out, extra="").replace(
+        "GaussElim       NoPivot", "GaussElim       Fastest"), tmp_path)
+    assert r.returncode == 1 and "GaussElim" in r.stderr
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("extra", ["DeviceLoop false", "DeviceLoop true",
-                                   "DeviceLoop true\nSweepEngine Queue"])
-def test_driver_reproduces_gold(tmp_path, gold, extra):
-    print(f"driver gold {extra}", flush=True)
+@pytest.mark.parametrize("extra,gauss", [
+    ("DeviceLoop false", "NoPivot"), ("DeviceLoop true", "NoPivot"),
+    ("DeviceLoop true\nSweepEngine Queue", "NoPivot"),
+    ("DeviceLoop true", "Original"), ("DeviceLoop true", "CramerGlu"),
+    ("DeviceLoop false", "CramerIntel")])
+def test_driver_reproduces_gold(tmp_path, gold, extra, gauss):
+    print(f"driver gold {extra} GaussElim {gauss}", flush=True)
     out = str(tmp_path / "out.psi")
-    r = run(GOLD_DECK.format(out=out, extra=extra), tmp_path)
+    deck = GOLD_DECK.format(out=out, extra=extra).replace(
+        "GaussElim       NoPivot", f"GaussElim       {gauss}")
+    r = run(deck, tmp_path)
     assert r.returncode == 0, r.stderr
     psi = psi_io.read_psi(out)
     assert psi.shape == (208, 128, 4, 2)

@@ -11,7 +11,9 @@ from tycho2_amd import mesh as meshmod
 from tycho2_amd import problems
 from tycho2_amd.psi_io import rel_diff
 from tycho2_amd.quadrature import product_quadrature
-from tycho2_amd.sweeper import FP32, FP64, SweeperHIP
+from tycho2_amd.sweeper import (FP32, FP64, GAUSS_CRAMER_GLU,
+                                GAUSS_CRAMER_INTEL, GAUSS_ORIGINAL,
+                                SweeperHIP, T2Error)
 
 pytestmark = pytest.mark.gpu
 
@@ -54,6 +56,29 @@ def test_single_sweep_fp32(ng):
         s.sweep(psi, src)
     ref, _ = orc.sweep(orc.build_mesh(sm), omega, ng, st, src)
     assert maxrel(psi, ref) <= 2e-5
+
+
+@pytest.mark.parametrize("gauss,ng,precision", [
+    (GAUSS_ORIGINAL, 4, FP64), (GAUSS_CRAMER_GLU, 16, FP64),
+    (GAUSS_CRAMER_INTEL, 1, FP64), (GAUSS_ORIGINAL, 8, FP32),
+    (GAUSS_CRAMER_GLU, 4, FP32)])
+def test_cell_solvers_match_oracle(gauss, ng, precision):
+    """The pivoting and Cramer cell solves against the oracle's same
+    method (the reference's GaussElim options)."""
+    print(f"cell solver {gauss} ng={ng} precision={precision}", flush=True)
+    sm, conn, omega, wt, st, ss, rng = random_problem(ng=ng, seed=100 + ng)
+    src = rng.uniform(-0.5, 2.0, (sm.n_cells, omega.shape[0], 4, ng))
+    psi = np.zeros_like(src)
+    with SweeperHIP(sm, conn, omega, wt, ng, st, ss, precision,
+                    gauss=gauss) as s:
+        with pytest.raises(T2Error):
+            s.set_engine(1)
+        if ng > 1:
+            with pytest.raises(T2Error):
+                s.set_launch(0, 2)
+        s.sweep(psi, src)
+    ref, _ = orc.sweep(orc.build_mesh(sm), omega, ng, st, src, method=gauss)
+    assert maxrel(psi, ref) <= (1e-12 if precision == FP64 else 2e-5)
 
 
 def test_launch_shapes_bitwise():

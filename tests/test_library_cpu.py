@@ -117,8 +117,8 @@ def test_create_rejects_bad_arguments():
     assert rc == -1 and "power of two" in msg
     rc, msg = _create(precision=7)
     assert rc == -1 and "precision" in msg
-    rc, msg = _create(gauss=2)
-    assert rc == -1 and "NoPivot" in msg
+    rc, msg = _create(gauss=4)
+    assert rc == -1 and "T2_GAUSS" in msg
 
 
 def test_null_handle_errors():

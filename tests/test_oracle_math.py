@@ -30,8 +30,11 @@ def oracle_local(P, omega, sigma, q, upwind, method=orc.NOPIVOT):
     return orc.cell_solve(vol, abar, sigma, q, up, method)
 
 
+METHODS = [orc.NOPIVOT, orc.ORIGINAL, orc.CRAMER_GLU, orc.CRAMER_INTEL]
+
+
 @pytest.mark.parametrize("seed", range(20))
-@pytest.mark.parametrize("method", [orc.NOPIVOT, orc.ORIGINAL])
+@pytest.mark.parametrize("method", METHODS)
 def test_cell_solve_matches_weak_form(seed, method):
     rng = np.random.default_rng(seed)
     P = random_tet(rng)
@@ -46,7 +49,8 @@ def test_cell_solve_matches_weak_form(seed, method):
     assert np.allclose(got, want, rtol=1e-11, atol=1e-12)
 
 
-def test_constant_preserved():
+@pytest.mark.parametrize("method", METHODS)
+def test_constant_preserved(method):
     """psi = c with inflow c and q = sigma c is reproduced exactly (the
     streaming of a constant integrates to zero)."""
     rng = np.random.default_rng(1)
@@ -56,11 +60,12 @@ def test_constant_preserved():
         omega /= np.linalg.norm(omega)
         sigma, c = rng.uniform(0.2, 3.0), rng.uniform(0.5, 2.0)
         got = oracle_local(P, omega, sigma, np.full(4, sigma * c),
-                           [np.full(4, c)] * 4)
+                           [np.full(4, c)] * 4, method)
         assert np.allclose(got, c, rtol=1e-13)
 
 
-def test_linear_solution_exact():
+@pytest.mark.parametrize("method", METHODS)
+def test_linear_solution_exact(method):
     """A linear psi with q = Omega.grad psi + sigma psi and exact inflow
     traces is reproduced at the vertices."""
     rng = np.random.default_rng(2)
@@ -72,11 +77,13 @@ def test_linear_solution_exact():
         alpha, beta = rng.normal(), rng.normal(size=3)
         exact = alpha + P @ beta
         q = omega @ beta + sigma * exact
-        got = oracle_local(P, omega, sigma, q, [exact] * 4)
+        got = oracle_local(P, omega, sigma, q, [exact] * 4, method)
         assert np.allclose(got, exact, rtol=1e-12, atol=1e-12)
 
 
-def test_pivot_and_nopivot_agree():
+def test_solvers_agree():
+    """Elimination with and without pivoting and the adjugate solve agree
+    to rounding; the two Cramer deck values are the same solve."""
     rng = np.random.default_rng(3)
     for _ in range(50):
         P = random_tet(rng)
@@ -85,8 +92,11 @@ def test_pivot_and_nopivot_agree():
         args = (P, omega, rng.uniform(0.1, 10), rng.uniform(0, 1, 4),
                 [rng.uniform(0, 1, 4) for _ in range(4)])
         a = oracle_local(*args, method=orc.NOPIVOT)
-        b = oracle_local(*args, method=orc.ORIGINAL)
-        assert np.allclose(a, b, rtol=1e-12, atol=1e-14)
+        for m in (orc.ORIGINAL, orc.CRAMER_GLU):
+            b = oracle_local(*args, method=m)
+            assert np.allclose(a, b, rtol=1e-12, atol=1e-14)
+        assert np.array_equal(oracle_local(*args, method=orc.CRAMER_GLU),
+                              oracle_local(*args, method=orc.CRAMER_INTEL))
 
 
 @pytest.mark.parametrize("sn", [2, 4, 8, 12, 16])

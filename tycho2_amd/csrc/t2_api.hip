@@ -149,6 +149,18 @@ void dfree(t2_sweeper *s, P *&p, size_t bytes)
     p = nullptr;
 }
 
+// The deck's GaussElim choice as the kernels' solver (both Cramer options
+// are one adjugate solve).
+int cell_solver(int gauss)
+{
+    switch (gauss) {
+    case T2_GAUSS_ORIGINAL: return t2::kSolveOriginal;
+    case T2_GAUSS_CRAMER_GLU:
+    case T2_GAUSS_CRAMER_INTEL: return t2::kSolveCramer;
+    default: return t2::kSolveNoPivot;
+    }
+}
+
 int set_device(const t2_sweeper *s)
 {
     T2_HIP(hipSetDevice(s->device));
@@ -214,6 +226,7 @@ int enqueue_sweep(t2_sweeper *s, const double *ang, const void *ang_t)
     a.src_ang_t = static_cast<const T *>(ang_t);
     a.psi = static_cast<T *>(s->d_psi);
     a.lag_buf = static_cast<const T *>(s->d_lag_buf);
+    a.solver = cell_solver(s->gauss);
     const auto *geo = static_cast<const t2::ItemGeo<T> *>(s->d_item_geo);
     // per-step launches: the whole schedule (level engine) or the steps
     // before the queued tail (hybrid)
@@ -931,8 +944,9 @@ int t2_sweeper_create(t2_sweeper **out, int device, int64_t n_cells,
         return fail(T2_ERR_ARG, "n_groups must be a power of two <= 64");
     if (precision != T2_FP64 && precision != T2_FP32)
         return fail(T2_ERR_ARG, "precision must be T2_FP64 or T2_FP32");
-    if (gauss != T2_GAUSS_NOPIVOT)
-        return fail(T2_ERR_ARG, "only GaussElim NoPivot is implemented");
+    if (gauss != T2_GAUSS_NOPIVOT && gauss != T2_GAUSS_ORIGINAL &&
+        gauss != T2_GAUSS_CRAMER_GLU && gauss != T2_GAUSS_CRAMER_INTEL)
+        return fail(T2_ERR_ARG, "gauss must be one of T2_GAUSS_*");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         (void)hipGetLastError();
@@ -1025,6 +1039,9 @@ int t2_set_launch(t2_sweeper *s, int items_per_block, int groups_per_lane)
     int gpl = groups_per_lane ? groups_per_lane : s->gpl;
     if (gpl != 1 && gpl != 2 && gpl != 4)
         return fail(T2_ERR_ARG, "groups_per_lane must be 1, 2 or 4");
+    if (gpl != 1 && s->gauss != T2_GAUSS_NOPIVOT)
+        return fail(T2_ERR_ARG, "the pivoting and Cramer cell solvers run "
+                                "one group per lane");
     if (gpl > s->ng)
         return fail(T2_ERR_ARG, "groups_per_lane exceeds the group count");
     int block = s->block;
@@ -1387,6 +1404,9 @@ int t2_set_engine(t2_sweeper *s, int engine, int chunk_rounds,
         return fail(T2_ERR_ARG, "spin_limit < 0");
     if (chunk_rounds < 0 || chunk_rounds > 256)
         return fail(T2_ERR_ARG, "chunk_rounds must be in [0, 256]");
+    if (engine != T2_ENGINE_LEVELS && s->gauss != T2_GAUSS_NOPIVOT)
+        return fail(T2_ERR_ARG, "the ready-queue engines use the NoPivot "
+                                "cell solver");
     const int rounds = chunk_rounds ? chunk_rounds : 4;
     T2_HIP(hipStreamSynchronize(s->stream));
     if (engine != s->engine || spin_limit != s->spin_limit)
@@ -1891,6 +1911,9 @@ int t2_batch_create(t2_batch **out, t2_sweeper *const *members, int count)
                                     "precision");
         if (m->d_qang)
             return fail(T2_ERR_ARG, "batch members need isotropic sources");
+        if (m->gauss != T2_GAUSS_NOPIVOT)
+            return fail(T2_ERR_ARG, "batch members use the NoPivot cell "
+                                    "solver");
         for (int j = 0; j < i; j++)
             if (members[j] == m)
                 return fail(T2_ERR_ARG, "a handle appears twice in a batch");

@@ -94,6 +94,103 @@ __device__ __forceinline__ void solve_nopivot(T m[4][4], T b[4])
             b[r] -= m[r][c] * b[c];
 }
 
+// 4x4 solve by elimination with partial pivoting (src/Transport.cc:284-334):
+// the first row holding the largest |pivot| is swapped in, then a
+// column-by-column LU and the two triangular solves.  Rows are swapped by
+// selects over compile-time indices (no dynamically indexed registers).
+template <typename T>
+__device__ __forceinline__ void solve_pivot(T m[4][4], T b[4])
+{
+#pragma unroll
+    for (int col = 0; col < 3; col++) {
+        int best = col;
+        T big = m[col][col] < T(0) ? -m[col][col] : m[col][col];
+#pragma unroll
+        for (int row = col + 1; row < 4; row++) {
+            const T t = m[row][col] < T(0) ? -m[row][col] : m[row][col];
+            if (t > big) {
+                best = row;
+                big = t;
+            }
+        }
+#pragma unroll
+        for (int row = col + 1; row < 4; row++) {
+            const bool sw = best == row;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const T x = m[row][c], y = m[col][c];
+                m[row][c] = sw ? y : x;
+                m[col][c] = sw ? x : y;
+            }
+            const T x = b[row], y = b[col];
+            b[row] = sw ? y : x;
+            b[col] = sw ? x : y;
+        }
+        m[col][col] = recip<T>(m[col][col]);
+#pragma unroll
+        for (int row = col + 1; row < 4; row++)
+            m[row][col] *= m[col][col];
+#pragma unroll
+        for (int k = 0; k <= col; k++)
+#pragma unroll
+            for (int row = k + 1; row < 4; row++)
+                m[row][col + 1] -= m[row][k] * m[k][col + 1];
+    }
+    m[3][3] = recip<T>(m[3][3]);
+#pragma unroll
+    for (int col = 0; col < 3; col++)
+#pragma unroll
+        for (int row = col + 1; row < 4; row++)
+            b[row] -= m[row][col] * b[col];
+#pragma unroll
+    for (int col = 3; col >= 0; col--) {
+        b[col] *= m[col][col];
+#pragma unroll
+        for (int row = col - 1; row >= 0; row--)
+            b[row] -= m[row][col] * b[col];
+    }
+}
+
+// Cramer's rule x = adj(A) b / det(A) (the reference's CramerGlu and
+// CramerIntel options, src/Transport.cc:414-673), the adjugate by Laplace
+// expansion over complementary 2x2 minors of rows {0,1} and {2,3}
+// (D. Eberly, "The Laplace Expansion Theorem", Geometric Tools, 2008).
+template <typename T>
+__device__ __forceinline__ void solve_cramer(const T a[4][4], T b[4])
+{
+    const T s0 = a[0][0] * a[1][1] - a[1][0] * a[0][1];
+    const T s1 = a[0][0] * a[1][2] - a[1][0] * a[0][2];
+    const T s2 = a[0][0] * a[1][3] - a[1][0] * a[0][3];
+    const T s3 = a[0][1] * a[1][2] - a[1][1] * a[0][2];
+    const T s4 = a[0][1] * a[1][3] - a[1][1] * a[0][3];
+    const T s5 = a[0][2] * a[1][3] - a[1][2] * a[0][3];
+    const T c5 = a[2][2] * a[3][3] - a[3][2] * a[2][3];
+    const T c4 = a[2][1] * a[3][3] - a[3][1] * a[2][3];
+    const T c3 = a[2][1] * a[3][2] - a[3][1] * a[2][2];
+    const T c2 = a[2][0] * a[3][3] - a[3][0] * a[2][3];
+    const T c1 = a[2][0] * a[3][2] - a[3][0] * a[2][2];
+    const T c0 = a[2][0] * a[3][1] - a[3][0] * a[2][1];
+    const T inv = recip<T>(s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 +
+                           s5 * c0);
+    const T b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3];
+    b[0] = ((a[1][1] * c5 - a[1][2] * c4 + a[1][3] * c3) * b0 +
+            (-a[0][1] * c5 + a[0][2] * c4 - a[0][3] * c3) * b1 +
+            (a[3][1] * s5 - a[3][2] * s4 + a[3][3] * s3) * b2 +
+            (-a[2][1] * s5 + a[2][2] * s4 - a[2][3] * s3) * b3) * inv;
+    b[1] = ((-a[1][0] * c5 + a[1][2] * c2 - a[1][3] * c1) * b0 +
+            (a[0][0] * c5 - a[0][2] * c2 + a[0][3] * c1) * b1 +
+            (-a[3][0] * s5 + a[3][2] * s2 - a[3][3] * s1) * b2 +
+            (a[2][0] * s5 - a[2][2] * s2 + a[2][3] * s1) * b3) * inv;
+    b[2] = ((a[1][0] * c4 - a[1][1] * c2 + a[1][3] * c0) * b0 +
+            (-a[0][0] * c4 + a[0][1] * c2 - a[0][3] * c0) * b1 +
+            (a[3][0] * s4 - a[3][1] * s2 + a[3][3] * s0) * b2 +
+            (-a[2][0] * s4 + a[2][1] * s2 - a[2][3] * s0) * b3) * inv;
+    b[3] = ((-a[1][0] * c3 + a[1][1] * c1 - a[1][2] * c0) * b0 +
+            (a[0][0] * c3 - a[0][1] * c1 + a[0][2] * c0) * b1 +
+            (-a[3][0] * s3 + a[3][1] * s1 - a[3][2] * s0) * b2 +
+            (a[2][0] * s3 - a[2][1] * s1 + a[2][2] * s0) * b3) * inv;
+}
+
 // ---------------------------------------------------------------- set-up
 template <typename T>
 __global__ void build_items_kernel(const uint32_t *__restrict__ work,
@@ -162,7 +259,8 @@ __global__ void build_items_kernel(const uint32_t *__restrict__ work,
 // read and psi written at the device coherence point (sc1 loads, write-
 // through stores), as the ready-queue kernel needs; otherwise plain cached
 // accesses (kernel boundaries order the per-step launches).
-template <typename T, int G, int GPL, bool COH = false>
+template <typename T, int G, int GPL, bool COH = false,
+          int SOLVER = kSolveNoPivot>
 __device__ __forceinline__ void solve_item(const SweepArgs<T> &p,
                                            int64_t item, int gl,
                                            const ItemTopo &t)
@@ -278,7 +376,12 @@ __device__ __forceinline__ void solve_item(const SweepArgs<T> &p,
             for (int j = 0; j < 4; j++)
                 m[i][j] = (i == j) ? ge_a12[i] + T(2) * (s + (P - o[i]))
                                    : ge_a12[i] + s + ((P - o[i]) - o[j]);
-        solve_nopivot<T>(m, b);
+        if constexpr (SOLVER == kSolveOriginal)
+            solve_pivot<T>(m, b);
+        else if constexpr (SOLVER == kSolveCramer)
+            solve_cramer<T>(m, b);
+        else
+            solve_nopivot<T>(m, b);
 
         T *dst = psi + size_t(t.angle) * plane + own + g;
 #pragma unroll
@@ -292,7 +395,7 @@ __device__ __forceinline__ void solve_item(const SweepArgs<T> &p,
     }
 }
 
-template <typename T, int G, int GPL>
+template <typename T, int G, int GPL, int SOLVER = kSolveNoPivot>
 __global__ void __launch_bounds__(kMaxBlock)
 sweep_step_kernel(const SweepArgs<T> p)
 {
@@ -311,7 +414,7 @@ sweep_step_kernel(const SweepArgs<T> p)
     const ItemTopo t = p.topo[item];
     if (p.active && !p.active[t.problem])
         return;
-    solve_item<T, G, GPL>(p, item, gl, t);
+    solve_item<T, G, GPL, false, SOLVER>(p, item, gl, t);
 }
 
 // ---------------------------------------------------------------- ready queue
@@ -567,6 +670,20 @@ hipError_t launch_g(const SweepArgs<T> &args, int gpl, int block,
     const int64_t threads = args.n_items * lanes_per_item;
     const int64_t blocks = (threads + block - 1) / block;
     const unsigned grid = unsigned((blocks + kXcds - 1) / kXcds * kXcds);
+    if (args.solver != kSolveNoPivot) {
+        // the other solvers run one group per lane
+        if (gpl != 1)
+            return hipErrorInvalidValue;
+        if (args.solver == kSolveOriginal)
+            hipLaunchKernelGGL((sweep_step_kernel<T, G, 1, kSolveOriginal>),
+                               dim3(grid), dim3(block), 0, stream, args);
+        else if (args.solver == kSolveCramer)
+            hipLaunchKernelGGL((sweep_step_kernel<T, G, 1, kSolveCramer>),
+                               dim3(grid), dim3(block), 0, stream, args);
+        else
+            return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
     switch (gpl) {
     case 1:
         hipLaunchKernelGGL((sweep_step_kernel<T, G, 1>), dim3(grid),

@@ -57,6 +57,10 @@ struct ProblemDesc {
 
 enum SigmaMode { kSigmaCellGroup = 0, kSigmaGroup = 1, kSigmaCell = 2 };
 
+// Cell solvers: elimination without pivoting (0, what a zero-initialised
+// SweepArgs selects), with partial pivoting, and Cramer's rule (adjugate).
+enum CellSolver { kSolveNoPivot = 0, kSolveOriginal = 1, kSolveCramer = 2 };
+
 template <typename T>
 struct SweepArgs {
     const ItemGeo<T> *geo;     // step slice of the item records
@@ -75,6 +79,9 @@ struct SweepArgs {
     // whose active[] flag is 0 are skipped (active may be null: all)
     const ProblemDesc<T> *problems;
     const int32_t *active;
+    // CellSolver; the level engine's kernels take any, the ready-queue
+    // kernels kSolveNoPivot
+    int solver;
 };
 
 // Defect of the coupled transport equations for a stored angular flux,

@@ -280,12 +280,23 @@ public:
         const std::string engine = deck.str("SweepEngine", "Levels");
         if (engine != "Levels" && engine != "Queue")
             throw std::runtime_error("SweepEngine must be Levels or Queue");
+        // GaussElim as the reference reads it (src/Main.cc:167-176)
+        const std::string ge = deck.str("GaussElim", "NoPivot");
+        int gauss = T2_GAUSS_NOPIVOT;
+        if (ge == "Original")
+            gauss = T2_GAUSS_ORIGINAL;
+        else if (ge == "CramerGlu")
+            gauss = T2_GAUSS_CRAMER_GLU;
+        else if (ge == "CramerIntel")
+            gauss = T2_GAUSS_CRAMER_INTEL;
+        else if (ge != "NoPivot")
+            throw std::runtime_error("GaussElim not recognized: " + ge);
         check(t2_sweeper_create(&c_engine, 0, p.mesh.nc, p.mesh.nn,
                                 p.mesh.coords.data(), p.mesh.cell_nodes.data(),
                                 p.mesh.adj.data(), p.mesh.f2c.data(), p.na,
                                 p.omega.data(), p.wt.data(), p.ng,
                                 p.sigma_t.data(), p.sigma_s.data(), prec,
-                                T2_GAUSS_NOPIVOT));
+                                gauss));
         check(t2_set_engine(c_engine,
                             engine == "Queue" ? T2_ENGINE_QUEUE
                                               : T2_ENGINE_LEVELS,

@@ -20,7 +20,10 @@ from . import build as _build
 
 FP64 = 0
 FP32 = 1
+GAUSS_ORIGINAL = 0
 GAUSS_NOPIVOT = 1
+GAUSS_CRAMER_GLU = 2
+GAUSS_CRAMER_INTEL = 3
 ENGINE_LEVELS = 0
 ENGINE_QUEUE = 1
 ENGINE_HYBRID = 2
@@ -194,7 +197,8 @@ class SweeperHIP:
     """Device sweeper for one mesh, quadrature and cross-section set."""
 
     def __init__(self, smesh, conn, omega, wt, ng, sigma_t, sigma_s,
-                 precision: int = FP64, device: int = 0):
+                 precision: int = FP64, device: int = 0,
+                 gauss: int = GAUSS_NOPIVOT):
         nc = smesh.n_cells
         self.nc, self.na, self.ng = nc, int(np.shape(omega)[0]), int(ng)
         st = _f64(np.broadcast_to(sigma_t, (nc, ng)))
@@ -206,7 +210,7 @@ class SweeperHIP:
         _check(lib().t2_sweeper_create(
             ctypes.byref(h), device, nc, smesh.n_nodes, _ptr(xyz),
             _ptr(cn, _ip), _ptr(adj, _ip), _ptr(f2c, _ip), self.na, _ptr(om),
-            _ptr(w), self.ng, _ptr(st), _ptr(ss), precision, GAUSS_NOPIVOT))
+            _ptr(w), self.ng, _ptr(st), _ptr(ss), precision, int(gauss)))
         self._h = h
 
     @classmethod
