@@ -74,3 +74,31 @@ def test_driver_reproduces_gold(tmp_path, gold, extra, gauss):
     l1, linf = psi_io.rel_diff(got, gold["psi_g0"])
     assert l1 < 1e-10 and linf < 1e-10
     assert "iterations: 34" in r.stdout
+
+
+@pytest.mark.gpu
+def test_driver_reads_partitioned_mesh(tmp_path, gold):
+    """The gold deck on cube-208 written as a 3-partition .pmesh (the
+    reference executable's mesh input) reproduces gold.psi."""
+    from tycho2_amd import mesh as meshmod
+    print("driver gold from .pmesh", flush=True)
+    sm = meshmod.read_smesh(golden("cube-208.smesh"))
+    pm = str(tmp_path / "cube-208.pmesh")
+    meshmod.write_pmesh(sm, pm, np.arange(sm.n_cells) % 3, 3)
+    out = str(tmp_path / "out.psi")
+    r = run(GOLD_DECK.format(out=out, extra="DeviceLoop true"), tmp_path,
+            mesh=pm)
+    assert r.returncode == 0, r.stderr
+    psi = psi_io.read_psi(out)
+    got = psi[..., 0][:, gold["psi_angles"], :]
+    l1, linf = psi_io.rel_diff(got, gold["psi_g0"])
+    assert l1 < 1e-10 and linf < 1e-10
+
+
+def test_driver_rejects_bad_mesh(tmp_path):
+    bad = tmp_path / "bad.pmesh"
+    bad.write_bytes(b"Tycho 2 Parallel Mesh".ljust(32, b"\0") + b"\1" * 8)
+    out = str(tmp_path / "o.psi")
+    r = run(GOLD_DECK.format(out=out, extra=""), tmp_path, mesh=str(bad))
+    assert r.returncode == 1 and "parallel mesh" in r.stderr
+

@@ -87,3 +87,73 @@ def test_rel_diff_metric():
     l1, linf = psi_io.rel_diff(a, b)
     assert l1 == pytest.approx(0.3 / 7.0)
     assert linf == pytest.approx(0.2 / 4.0)
+
+
+# ---------------------------------------------------------------- .pmesh
+REF_PMESH = os.path.join(os.path.dirname(orc.__file__), "_ref",
+                         "libref_pmesh.so")
+
+
+def partitions(sm, kind):
+    """Per-cell partition numbers: one partition, round robin, or slabs of
+    the x coordinate (the reference's PartitionColumns cuts columns)."""
+    if kind == "one":
+        return np.zeros(sm.n_cells, np.int64), 1
+    if kind == "robin":
+        return np.arange(sm.n_cells) % 3, 3
+    x = sm.coords[sm.cell_nodes].mean(axis=1)[:, 0]
+    edges = np.quantile(x, [0.25, 0.5, 0.75])
+    return np.searchsorted(edges, x).astype(np.int64), 4
+
+
+@pytest.mark.parametrize("kind", ["one", "robin", "slabs"])
+def test_pmesh_round_trip(tmp_path, kind):
+    sm = meshmod.read_smesh(golden("cube-208.smesh"))
+    part, n = partitions(sm, kind)
+    path = str(tmp_path / "m.pmesh")
+    meshmod.write_pmesh(sm, path, part, n)
+    back = meshmod.read_pmesh(path)
+    for k in ("cell_faces", "cell_nodes", "material", "face_cells",
+              "face_nodes", "coords"):
+        assert np.array_equal(getattr(back, k), getattr(sm, k)), k
+
+
+def test_pmesh_rejects_bad_files(tmp_path):
+    sm = meshmod.read_smesh(golden("cube-12.smesh"))
+    path = tmp_path / "m.pmesh"
+    meshmod.write_pmesh(sm, str(path))
+    raw = path.read_bytes()
+    (tmp_path / "short.pmesh").write_bytes(raw[:-8])
+    with pytest.raises(ValueError):
+        meshmod.read_pmesh(str(tmp_path / "short.pmesh"))
+    (tmp_path / "name.pmesh").write_bytes(b"X" + raw[1:])
+    with pytest.raises(ValueError):
+        meshmod.read_pmesh(str(tmp_path / "name.pmesh"))
+
+
+@pytest.mark.skipif(not os.path.exists(REF_PMESH),
+                    reason="oracle/_ref not built (no /root/reference here)")
+@pytest.mark.parametrize("kind", ["one", "robin", "slabs"])
+@pytest.mark.parametrize("name", ["cube-12.smesh", "cube-208.smesh"])
+def test_pmesh_bytes_equal_reference_writer(tmp_path, kind, name):
+    """The package's .pmesh bytes equal what the reference's own
+    ParallelMesh::createFromSerialMesh + write produce from the same serial
+    mesh and partition vector (built from its sources into oracle/_ref)."""
+    import ctypes
+    ref = ctypes.CDLL(REF_PMESH)
+    ref.ref_smesh_to_pmesh.restype = ctypes.c_int64
+    ref.ref_smesh_to_pmesh.argtypes = [
+        ctypes.c_char_p, ctypes.c_char_p,
+        ctypes.POINTER(ctypes.c_uint64), ctypes.c_int64, ctypes.c_int]
+    src = golden(name)
+    sm = meshmod.read_smesh(src)
+    part, n = partitions(sm, kind)
+    pu = np.ascontiguousarray(part, np.uint64)
+    want = tmp_path / "ref.pmesh"
+    rc = ref.ref_smesh_to_pmesh(
+        src.encode(), str(want).encode(),
+        pu.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), sm.n_cells, n)
+    assert rc == sm.n_cells
+    got = tmp_path / "ours.pmesh"
+    meshmod.write_pmesh(sm, str(got), part, n)
+    assert got.read_bytes() == want.read_bytes()

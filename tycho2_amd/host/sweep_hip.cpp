@@ -11,7 +11,8 @@
 // "Precision FP32" and "SweepEngine Queue" select the engine's FP32 sweeps
 // and its persistent ready-queue launch.
 //
-// Usage: sweep_hip.x <mesh.smesh> <input.deck>
+// Usage: sweep_hip.x <mesh.smesh | mesh.pmesh> <input.deck>
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -102,17 +103,37 @@ struct Mesh {
     std::vector<int64_t> adj, f2c;
 };
 
-Mesh read_smesh(const std::string &path)
+// Adjacency and face-to-cell vertex maps from the cell/face records.
+void build_connectivity(Mesh &m)
 {
-    std::ifstream in(path, std::ios::binary);
-    if (!in)
-        throw std::runtime_error("cannot open mesh " + path);
-    char name[32];
+    m.adj.assign(4 * m.nc, -1);
+    m.f2c.assign(12 * m.nc, -1);
+    for (int64_t c = 0; c < m.nc; c++) {
+        const int64_t *cn = &m.cell_nodes[4 * c];
+        for (int s = 0; s < 4; s++) {
+            const int64_t f = m.cell_faces[4 * c + s];
+            const int64_t *fn = &m.face_nodes[3 * f];
+            int lf = -1;
+            for (int v = 0; v < 4; v++)
+                if (cn[v] != fn[0] && cn[v] != fn[1] && cn[v] != fn[2])
+                    lf = v;
+            if (lf < 0)
+                throw std::runtime_error("mesh: a face is not a cell face");
+            const int64_t c0 = m.face_cells[2 * f], c1 = m.face_cells[2 * f + 1];
+            m.adj[4 * c + lf] = c1 < 0 ? -1 : (c0 == c ? c1 : c0);
+            for (int j = 0; j < 3; j++)
+                for (int v = 0; v < 4; v++)
+                    if (fn[j] == cn[v])
+                        m.f2c[(4 * c + lf) * 3 + j] = v;
+        }
+    }
+}
+
+Mesh read_smesh(std::ifstream &in, const std::string &path)
+{
     uint64_t hdr[4];
-    in.read(name, 32);
     in.read(reinterpret_cast<char *>(hdr), sizeof(hdr));
-    if (!in || std::strncmp(name, "Tycho 2 Serial Mesh", 32) != 0 ||
-        hdr[0] != 2)
+    if (!in || hdr[0] != 2)
         throw std::runtime_error(path + ": not a Tycho 2 serial mesh v2");
     Mesh m;
     m.nc = int64_t(hdr[1]);
@@ -140,25 +161,116 @@ Mesh read_smesh(const std::string &path)
         for (int j = 0; j < 3; j++)
             m.face_nodes[3 * f + j] = faces[5 * f + 2 + j];
     }
-    m.adj.assign(4 * m.nc, -1);
-    m.f2c.assign(12 * m.nc, -1);
-    for (int64_t c = 0; c < m.nc; c++) {
-        const int64_t *cn = &m.cell_nodes[4 * c];
-        for (int s = 0; s < 4; s++) {
-            const int64_t f = m.cell_faces[4 * c + s];
-            const int64_t *fn = &m.face_nodes[3 * f];
-            int lf = -1;
-            for (int v = 0; v < 4; v++)
-                if (cn[v] != fn[0] && cn[v] != fn[1] && cn[v] != fn[2])
-                    lf = v;
-            const int64_t c0 = m.face_cells[2 * f], c1 = m.face_cells[2 * f + 1];
-            m.adj[4 * c + lf] = c1 < 0 ? -1 : (c0 == c ? c1 : c0);
-            for (int j = 0; j < 3; j++)
-                for (int v = 0; v < 4; v++)
-                    if (fn[j] == cn[v])
-                        m.f2c[(4 * c + lf) * 3 + j] = v;
-        }
+    return m;
+}
+
+// .pmesh v2 (src/ParallelMesh.hh:43-81): every partition read and merged
+// into one mesh numbered by the global IDs the file stores (one GPU holds
+// the whole mesh; the reference gives each MPI rank one partition).
+Mesh read_pmesh(std::ifstream &in, const std::string &path)
+{
+    const uint64_t inval = ~uint64_t(0);
+    uint64_t hdr[2];
+    in.read(reinterpret_cast<char *>(hdr), sizeof(hdr));
+    if (!in || hdr[0] != 2)
+        throw std::runtime_error(path + ": not a Tycho 2 parallel mesh v2");
+    const uint64_t np = hdr[1];
+    std::vector<uint64_t> off(np + 1);
+    in.read(reinterpret_cast<char *>(off.data()), off.size() * 8);
+    struct Part {
+        std::vector<uint64_t> cell, face, node;
+    };
+    std::vector<Part> parts(np);
+    uint64_t nc = 0, nf = 0, nn = 0;
+    for (uint64_t p = 0; p < np; p++) {
+        in.seekg(std::streamoff(off[p]));
+        uint64_t n[3];
+        in.read(reinterpret_cast<char *>(n), sizeof(n));
+        Part &pt = parts[p];
+        pt.cell.resize(n[0] * 10);
+        pt.face.resize(n[1] * 9);
+        pt.node.resize(n[2] * 4);
+        in.read(reinterpret_cast<char *>(pt.cell.data()), pt.cell.size() * 8);
+        in.read(reinterpret_cast<char *>(pt.face.data()), pt.face.size() * 8);
+        in.read(reinterpret_cast<char *>(pt.node.data()), pt.node.size() * 8);
+        if (!in)
+            throw std::runtime_error(path + ": truncated");
+        nc += n[0];
+        for (uint64_t f = 0; f < n[1]; f++)
+            nf = std::max(nf, pt.face[9 * f + 5] + 1);
+        for (uint64_t v = 0; v < n[2]; v++)
+            nn = std::max(nn, pt.node[4 * v + 3] + 1);
     }
+    Mesh m;
+    m.nc = int64_t(nc);
+    m.nf = int64_t(nf);
+    m.nn = int64_t(nn);
+    m.cell_faces.assign(4 * nc, -1);
+    m.cell_nodes.assign(4 * nc, -1);
+    m.face_cells.assign(2 * nf, -1);
+    m.face_nodes.assign(3 * nf, -1);
+    m.coords.assign(3 * nn, 0.0);
+    auto gid = [&](uint64_t part, uint64_t local_cell) -> int64_t {
+        if (part >= np || local_cell * 10 + 8 >= parts[part].cell.size())
+            throw std::runtime_error(path + ": bad cell reference");
+        return int64_t(parts[part].cell[local_cell * 10 + 8]);
+    };
+    for (uint64_t p = 0; p < np; p++) {
+        const Part &pt = parts[p];
+        const uint64_t pc = pt.cell.size() / 10, pf = pt.face.size() / 9;
+        const uint64_t pn = pt.node.size() / 4;
+        for (uint64_t c = 0; c < pc; c++) {
+            const uint64_t *r = &pt.cell[10 * c];
+            const uint64_t g = r[8];
+            if (g >= nc)
+                throw std::runtime_error(path + ": bad cell global ID");
+            for (int i = 0; i < 4; i++) {
+                if (r[i] >= pf || r[4 + i] >= pn)
+                    throw std::runtime_error(path + ": bad cell record");
+                m.cell_faces[4 * g + i] = int64_t(pt.face[9 * r[i] + 5]);
+                m.cell_nodes[4 * g + i] = int64_t(pt.node[4 * r[4 + i] + 3]);
+            }
+        }
+        for (uint64_t f = 0; f < pf; f++) {
+            const uint64_t *r = &pt.face[9 * f];
+            const uint64_t g = r[5];
+            for (int s = 0; s < 2; s++)
+                m.face_cells[2 * g + s] =
+                    r[s] == inval ? -1 : gid(r[6 + s], r[s]);
+            for (int j = 0; j < 3; j++) {
+                if (r[2 + j] >= pn)
+                    throw std::runtime_error(path + ": bad face record");
+                m.face_nodes[3 * g + j] = int64_t(pt.node[4 * r[2 + j] + 3]);
+            }
+        }
+        for (uint64_t v = 0; v < pn; v++)
+            std::memcpy(&m.coords[3 * pt.node[4 * v + 3]], &pt.node[4 * v],
+                        3 * sizeof(double));
+    }
+    return m;
+}
+
+// A .smesh or a .pmesh file, told apart by its format name.
+Mesh read_mesh(const std::string &path)
+{
+    std::ifstream in(path, std::ios::binary);
+    if (!in)
+        throw std::runtime_error("cannot open mesh " + path);
+    char name[33] = {};
+    in.read(name, 32);
+    if (!in)
+        throw std::runtime_error(path + ": not a Tycho 2 mesh");
+    Mesh m;
+    if (std::strncmp(name, "Tycho 2 Serial Mesh", 32) == 0)
+        m = read_smesh(in, path);
+    else if (std::strncmp(name, "Tycho 2 Parallel Mesh", 32) == 0)
+        m = read_pmesh(in, path);
+    else
+        throw std::runtime_error(path + ": not a Tycho 2 mesh");
+    for (int64_t f : m.cell_faces)
+        if (f < 0 || f >= m.nf)
+            throw std::runtime_error(path + ": bad face index");
+    build_connectivity(m);
     return m;
 }
 
@@ -403,7 +515,7 @@ void write_psi(const std::string &path, const PsiData &psi)
 int main(int argc, char **argv)
 {
     if (argc < 3) {
-        std::printf("Usage: ./sweep_hip.x <serial mesh> <input deck>\n");
+        std::printf("Usage: ./sweep_hip.x <serial or parallel mesh> <input deck>\n");
         return 0;
     }
     try {
@@ -413,7 +525,7 @@ int main(int argc, char **argv)
             throw std::runtime_error("Sweep type not recognized: " + type +
                                      " (this executable runs SweepType HIP)");
         Problem p;
-        p.mesh = read_smesh(argv[1]);
+        p.mesh = read_mesh(argv[1]);
         p.sn = int(deck.num("snOrder"));
         p.ng = int64_t(deck.num("nGroups"));
         quadrature(p.sn, p.omega, p.wt);

@@ -6,6 +6,10 @@
   (reference: src/SerialMesh.hh:43-68).
 * ``read_smesh`` / ``write_smesh`` use the reference's ``.smesh`` v2 binary
   layout (reference: src/SerialMesh.cc:209-293 for the reader).
+* ``write_pmesh`` / ``read_pmesh`` use its partitioned ``.pmesh`` v2 layout
+  (src/ParallelMesh.hh:43-81, src/ParallelMesh.cc:167-392): a serial mesh cut
+  into partitions by a per-cell partition number, and any such file read
+  back into one serial mesh (one GPU holds every partition).
 * ``kuhn_cube`` generates the unit-cube meshes of the benchmark problems: an
   ``n x n x n`` grid of hexahedra, each split into six tetrahedra around the
   (0,0,0)-(1,1,1) diagonal, with optional seeded jitter of interior nodes.
@@ -28,6 +32,11 @@ import numpy as np
 
 SMESH_NAME = b"Tycho 2 Serial Mesh"
 SMESH_VERSION = 2
+PMESH_NAME = b"Tycho 2 Parallel Mesh"
+PMESH_VERSION = 2
+INVALID = np.uint64(0xFFFFFFFFFFFFFFFF)
+# face boundary types of a parallel mesh (src/ParallelMesh.cc:88-90)
+MESH_BOUNDARY, PARTITION_BOUNDARY, NOT_BOUNDARY = 0, 1, 2
 
 
 @dataclasses.dataclass
@@ -93,6 +102,127 @@ def read_smesh(path: str) -> SerialMesh:
                       face_cells=faces[:, 0:2].copy(),
                       face_nodes=faces[:, 2:5].copy(),
                       coords=coords.copy())
+
+
+def write_pmesh(mesh: SerialMesh, path: str, partition=None,
+                n_parts: int | None = None) -> None:
+    """Write ``mesh`` as a ``.pmesh`` v2 file, cell c in partition
+    ``partition[c]`` (default: one partition).  Within a partition, cells,
+    faces and nodes keep ascending global order; a face's bounding cell in
+    another partition is that partition's local index, its partition entry
+    the other partition, its type PartitionBoundary; a face on the domain
+    boundary has INVALID for its missing cell and partition, type
+    MeshBoundary; any other face NotBoundary."""
+    nc = mesh.n_cells
+    part = (np.zeros(nc, np.int64) if partition is None
+            else np.asarray(partition, np.int64))
+    n_parts = int(part.max()) + 1 if n_parts is None else int(n_parts)
+    cells_of = [np.flatnonzero(part == p) for p in range(n_parts)]
+    local = np.empty(nc, np.int64)           # cell -> index in its partition
+    for cs in cells_of:
+        local[cs] = np.arange(cs.size)
+    blobs = []
+    for p, cs in enumerate(cells_of):
+        faces = np.unique(mesh.cell_faces[cs])
+        nodes = np.unique(mesh.cell_nodes[cs])
+        cell = np.empty((cs.size, 10), np.uint64)
+        cell[:, 0:4] = np.searchsorted(faces, mesh.cell_faces[cs])
+        cell[:, 4:8] = np.searchsorted(nodes, mesh.cell_nodes[cs])
+        cell[:, 8] = cs
+        cell[:, 9] = mesh.material[cs]
+        face = np.empty((faces.size, 9), np.uint64)
+        btype = np.full(faces.size, NOT_BOUNDARY, np.uint64)
+        for side in range(2):
+            gc = mesh.face_cells[faces, side]
+            edge = gc < 0
+            other = ~edge & (part[np.maximum(gc, 0)] != p)
+            face[:, side] = np.where(edge, INVALID,
+                                     local[np.maximum(gc, 0)].astype(np.uint64))
+            face[:, 6 + side] = np.where(
+                edge, INVALID, part[np.maximum(gc, 0)].astype(np.uint64))
+            btype = np.where(edge, np.uint64(MESH_BOUNDARY), btype)
+            btype = np.where(other, np.uint64(PARTITION_BOUNDARY), btype)
+        face[:, 2:5] = np.searchsorted(nodes, mesh.face_nodes[faces])
+        face[:, 5] = faces
+        face[:, 8] = btype
+        node = np.empty((nodes.size, 4), np.uint64)
+        node[:, 0:3] = np.ascontiguousarray(
+            mesh.coords[nodes], np.float64).view(np.uint64)
+        node[:, 3] = nodes
+        blobs.append(struct.pack("<3Q", cs.size, faces.size, nodes.size) +
+                     cell.tobytes() + face.tobytes() + node.tobytes())
+    head = 32 + (3 + n_parts) * 8
+    offsets = [head]
+    for b in blobs:
+        offsets.append(offsets[-1] + len(b))
+    with open(path, "wb") as f:
+        f.write(PMESH_NAME.ljust(32, b"\0"))
+        f.write(struct.pack(f"<{2 + n_parts + 1}Q", PMESH_VERSION, n_parts,
+                            *offsets))
+        for b in blobs:
+            f.write(b)
+
+
+def read_pmesh(path: str) -> SerialMesh:
+    """Read a ``.pmesh`` v2 file into one serial mesh numbered by the global
+    IDs it stores; raises ``ValueError`` on a bad header or inconsistent
+    partitions."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    if raw[:32].rstrip(b"\0") != PMESH_NAME:
+        raise ValueError(f"{path}: not a Tycho 2 parallel mesh")
+    version, n_parts = struct.unpack_from("<2Q", raw, 32)
+    if version != PMESH_VERSION:
+        raise ValueError(f"{path}: parallel mesh version {version} != 2")
+    offsets = struct.unpack_from(f"<{n_parts + 1}Q", raw, 48)
+    if offsets[-1] != len(raw):
+        raise ValueError(f"{path}: size mismatch")
+    parts = []
+    for p in range(n_parts):
+        off = offsets[p]
+        nc, nf, nn = struct.unpack_from("<3Q", raw, off)
+        off += 24
+        cell = np.frombuffer(raw, np.uint64, nc * 10, off).reshape(nc, 10)
+        off += nc * 80
+        face = np.frombuffer(raw, np.uint64, nf * 9, off).reshape(nf, 9)
+        off += nf * 72
+        node = np.frombuffer(raw, np.uint64, nn * 4, off).reshape(nn, 4)
+        if off + nn * 32 != offsets[p + 1]:
+            raise ValueError(f"{path}: partition {p} size mismatch")
+        parts.append((cell, face, node))
+    n_cells = sum(c.shape[0] for c, _, _ in parts)
+    n_faces = int(max(f[:, 5].max() for _, f, _ in parts if f.size)) + 1
+    n_nodes = int(max(n[:, 3].max() for _, _, n in parts if n.size)) + 1
+    cell_faces = np.full((n_cells, 4), -1, np.int64)
+    cell_nodes = np.full((n_cells, 4), -1, np.int64)
+    material = np.zeros(n_cells, np.int64)
+    face_cells = np.full((n_faces, 2), -2, np.int64)
+    face_nodes = np.full((n_faces, 3), -1, np.int64)
+    coords = np.full((n_nodes, 3), np.nan)
+    cell_gid = [c[:, 8].astype(np.int64) for c, _, _ in parts]
+    for p, (cell, face, node) in enumerate(parts):
+        fg = face[:, 5].astype(np.int64)
+        ng = node[:, 3].astype(np.int64)
+        g = cell_gid[p]
+        cell_faces[g] = fg[cell[:, 0:4].astype(np.int64)]
+        cell_nodes[g] = ng[cell[:, 4:8].astype(np.int64)]
+        material[g] = cell[:, 9].astype(np.int64)
+        coords[ng] = node[:, 0:3].copy().view(np.float64)
+        face_nodes[fg] = ng[face[:, 2:5].astype(np.int64)]
+        for side in range(2):
+            lc = face[:, side]
+            owner = face[:, 6 + side]
+            gc = np.full(fg.size, -1, np.int64)
+            for q in range(n_parts):
+                sel = (lc != INVALID) & (owner == np.uint64(q))
+                gc[sel] = cell_gid[q][lc[sel].astype(np.int64)]
+            face_cells[fg, side] = gc
+    if (cell_faces < 0).any() or (face_cells == -2).any() or \
+            np.isnan(coords).any():
+        raise ValueError(f"{path}: partitions do not cover the mesh")
+    return SerialMesh(cell_faces=cell_faces, cell_nodes=cell_nodes,
+                      material=material, face_cells=face_cells,
+                      face_nodes=face_nodes, coords=coords)
 
 
 # ---------------------------------------------------------------- generator
