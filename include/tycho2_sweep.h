@@ -119,6 +119,9 @@ int t2_sweeper_create(t2_sweeper **out, int device,
                       const double *omega, const double *weights,
                       int64_t n_groups, const double *sigma_t,
                       const double *sigma_s, int precision, int gauss);
+/* Environment: T2_NO_GRAPH=1 makes the level engine launch its per-step
+ * kernels directly instead of replaying a captured hipGraph (a debugging
+ * aid; results are identical). */
 int t2_sweeper_destroy(t2_sweeper *s);
 int t2_sweeper_info(const t2_sweeper *s, t2_info *info);
 

@@ -99,6 +99,7 @@ struct Deck {
 struct Mesh {
     int64_t nc = 0, nf = 0, nn = 0;
     std::vector<int64_t> cell_faces, cell_nodes, face_cells, face_nodes;
+    std::vector<int64_t> material;   // per cell
     std::vector<double> coords;
     std::vector<int64_t> adj, f2c;
 };
@@ -148,11 +149,14 @@ Mesh read_smesh(std::ifstream &in, const std::string &path)
         throw std::runtime_error(path + ": truncated");
     m.cell_faces.resize(4 * m.nc);
     m.cell_nodes.resize(4 * m.nc);
-    for (int64_t c = 0; c < m.nc; c++)
+    m.material.resize(m.nc);
+    for (int64_t c = 0; c < m.nc; c++) {
         for (int i = 0; i < 4; i++) {
             m.cell_faces[4 * c + i] = cells[9 * c + i];
             m.cell_nodes[4 * c + i] = cells[9 * c + 4 + i];
         }
+        m.material[c] = cells[9 * c + 8];
+    }
     m.face_cells.resize(2 * m.nf);
     m.face_nodes.resize(3 * m.nf);
     for (int64_t f = 0; f < m.nf; f++) {
@@ -207,6 +211,7 @@ Mesh read_pmesh(std::ifstream &in, const std::string &path)
     m.nn = int64_t(nn);
     m.cell_faces.assign(4 * nc, -1);
     m.cell_nodes.assign(4 * nc, -1);
+    m.material.assign(nc, 0);
     m.face_cells.assign(2 * nf, -1);
     m.face_nodes.assign(3 * nf, -1);
     m.coords.assign(3 * nn, 0.0);
@@ -224,6 +229,7 @@ Mesh read_pmesh(std::ifstream &in, const std::string &path)
             const uint64_t g = r[8];
             if (g >= nc)
                 throw std::runtime_error(path + ": bad cell global ID");
+            m.material[g] = int64_t(r[9]);
             for (int i = 0; i < 4; i++) {
                 if (r[i] >= pf || r[4 + i] >= pn)
                     throw std::runtime_error(path + ": bad cell record");
@@ -533,15 +539,7 @@ int main(int argc, char **argv)
         // per-material cross sections (src/Problem.cc:149-172)
         const double st[2] = {deck.num("sigmaT1"), deck.num("sigmaT2")};
         const double ss[2] = {deck.num("sigmaS1"), deck.num("sigmaS2")};
-        std::vector<int64_t> material(p.mesh.nc, 1);
-        {
-            std::ifstream in(argv[1], std::ios::binary);
-            std::vector<int64_t> cells(size_t(p.mesh.nc) * 9);
-            in.seekg(64);
-            in.read(reinterpret_cast<char *>(cells.data()), cells.size() * 8);
-            for (int64_t c = 0; c < p.mesh.nc; c++)
-                material[c] = cells[9 * c + 8];
-        }
+        const std::vector<int64_t> &material = p.mesh.material;
         p.sigma_t.resize(p.mesh.nc * p.ng);
         p.sigma_s.resize(p.mesh.nc * p.ng);
         for (int64_t c = 0; c < p.mesh.nc; c++) {
