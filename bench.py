@@ -295,9 +295,10 @@ def main() -> int:
                     help="only a short run for rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for "
-              f"--gpus {args.gpus}); the sharded design is described in "
-              f"DESIGN.md", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for --gpus "
+              f"{args.gpus}). Several GPUs would each sweep a share of the "
+              f"energy groups (groups are decoupled, so shards exchange only "
+              f"convergence norms); that path is not built.", file=sys.stderr)
         return 2
 
     from tycho2_amd import problems
