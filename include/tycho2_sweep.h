@@ -272,6 +272,14 @@ int t2_batch_solve_krylov(t2_batch *b, int64_t iter_max, double rtol,
 int t2_batch_solve_source_iteration(t2_batch *b, int64_t iter_max,
                                     double err_max, int64_t *n_iter,
                                     double *seconds);
+/* t2_solve_refined of every FP32 member, each of its stages in lock step
+ * (the correction's right-hand side sweep carries each member's own FP32
+ * residual source).  info and seconds (arrays of the member count, may be
+ * NULL): each member's t2_refine_info, and the wall time from the call to
+ * the end of its correction GMRES. */
+int t2_batch_solve_refined(t2_batch *b, int64_t iter_max, double rtol_sweep,
+                           double rtol_correction, int64_t restart,
+                           t2_refine_info *info, double *seconds);
 int t2_batch_synchronize(t2_batch *b);
 int t2_batch_get_stream(const t2_batch *b, void **stream);
 

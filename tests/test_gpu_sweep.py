@@ -366,6 +366,47 @@ def test_batch_solves_bitwise(precision, engine, method):
         s.close()
 
 
+@pytest.mark.parametrize("engine", [0, 1])
+def test_batch_refined_bitwise(engine):
+    """The lock-step refined solve (each member's correction sweep carries
+    its own FP32 residual source) gives every member its own refined solve's
+    phi, psi and counts bit for bit; a later batch sweep carries no residual
+    source."""
+    from tycho2_amd.sweeper import SweepBatch
+    print(f"batch refined engine={engine}", flush=True)
+    groups = (4, 16, 1)
+    own = batch_members(FP32, groups)
+    mem = batch_members(FP32, groups)
+    infos = [s.solve_refined(100, 1e-6, 1e-5, 10)[0] for s in own]
+    with SweepBatch(mem) as b:
+        b.set_engine(engine, 1 << 20)
+        got, sec = b.solve_refined(100, 1e-6, 1e-5, 10)
+        print(f"  {got}", flush=True)
+        assert got == infos
+        assert all(x > 0 for x in sec)
+        for s, m in zip(own, mem):
+            assert np.array_equal(s.phi(), m.phi())
+            assert np.array_equal(s.psi_angle(2), m.psi_angle(2))
+        for s in own:
+            s.sweep_step()
+        b.sweep_step()
+        b.synchronize()
+    for s, m in zip(own, mem):
+        assert np.array_equal(s.phi(), m.phi())
+    for s in own + mem:
+        s.close()
+
+
+def test_batch_refined_needs_fp32():
+    from tycho2_amd.sweeper import SweepBatch, T2Error
+    mem = batch_members(FP64)
+    with SweepBatch(mem) as b:
+        with pytest.raises(T2Error):
+            b.solve_refined(10)
+    for s in mem:
+        s.close()
+
+
 def test_batch_rejects_changed_member():
     from tycho2_amd.sweeper import SweepBatch, T2Error
     print("batch stale member", flush=True)

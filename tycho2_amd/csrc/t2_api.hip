@@ -757,6 +757,9 @@ struct Gmres {
         V = V_;
         hist = hist_;
         n = s->n_phi();
+        it = 0;
+        j = 0;
+        x_zero = true;
         double bb = 0.0;
         T2_TRY(dot(s, b, b, n, &bb));
         bnorm = std::sqrt(bb);
@@ -1509,6 +1512,7 @@ struct t2_batch {
     t2::BatchChunk *d_chunks = nullptr;
     unsigned long long *d_ticket = nullptr;
     int32_t *d_abort = nullptr;
+    std::vector<const void *> ang_t;     // members' T-typed angular sources
     int32_t *d_active = nullptr;         // members swept by the next sweep
     std::vector<int32_t> active;         // host copy (kept alive for copies)
     std::vector<int32_t> uploaded;       // flags the device holds
@@ -1588,6 +1592,79 @@ void drop_batch_graph(t2_batch *b)
     b->graph = nullptr;
 }
 
+template <typename T>
+t2::ProblemDesc<T> problem_desc(const t2_sweeper *m, const void *ang_t)
+{
+    t2::ProblemDesc<T> d{};
+    d.nc = m->nc;
+    d.sigma_mode = m->sigma_mode;
+    d.sigma = static_cast<const T *>(m->d_sigt);
+    d.src_iso = static_cast<const T *>(m->d_src);
+    d.src_ang_t = static_cast<const T *>(ang_t);
+    d.psi = static_cast<T *>(m->d_psi);
+    d.lag_buf = static_cast<const T *>(m->d_lag_buf);
+    return d;
+}
+
+template <typename T>
+t2::BatchProblem<T> batch_problem(const t2_sweeper *m, const void *ang_t)
+{
+    t2::BatchProblem<T> bp{};
+    bp.a.geo = static_cast<const t2::ItemGeo<T> *>(m->d_item_geo);
+    bp.a.topo = m->d_item_topo;
+    bp.a.n_items = m->nc * m->na;
+    bp.a.nc = m->nc;
+    bp.a.sigma_mode = m->sigma_mode;
+    bp.a.sigma = static_cast<const T *>(m->d_sigt);
+    bp.a.src_iso = static_cast<const T *>(m->d_src);
+    bp.a.src_ang_t = static_cast<const T *>(ang_t);
+    bp.a.psi = static_cast<T *>(m->d_psi);
+    bp.a.lag_buf = static_cast<const T *>(m->d_lag_buf);
+    bp.pending = m->d_pending;
+    bp.down = m->d_down;
+    bp.arm = m->d_arm;
+    bp.first_item = m->queue_item;
+    bp.groups = int32_t(m->ng);
+    return bp;
+}
+
+// Rewrites the device problem tables that exist (after a change of the
+// members' angle-dependent sources); the captured sweeps read them.
+template <typename T>
+int write_tables(t2_batch *b)
+{
+    for (auto &g : b->groups) {
+        if (!g.d_desc)
+            continue;
+        std::vector<t2::ProblemDesc<T>> desc(g.idx.size());
+        for (size_t j = 0; j < g.idx.size(); j++)
+            desc[j] = problem_desc<T>(b->members[g.idx[j]], b->ang_t[g.idx[j]]);
+        T2_HIP(hipMemcpy(g.d_desc, desc.data(),
+                         sizeof(t2::ProblemDesc<T>) * desc.size(),
+                         hipMemcpyHostToDevice));
+    }
+    if (b->d_problems) {
+        std::vector<t2::BatchProblem<T>> probs(b->members.size());
+        for (size_t i = 0; i < b->members.size(); i++)
+            probs[i] = batch_problem<T>(b->members[i], b->ang_t[i]);
+        T2_HIP(hipMemcpy(b->d_problems, probs.data(),
+                         sizeof(t2::BatchProblem<T>) * probs.size(),
+                         hipMemcpyHostToDevice));
+    }
+    return T2_OK;
+}
+
+// Sets every member's angle-dependent source for the following sweeps.
+int batch_set_ang_t(t2_batch *b, const std::vector<const void *> &ang_t)
+{
+    if (ang_t == b->ang_t)
+        return T2_OK;
+    T2_HIP(hipStreamSynchronize(b->stream));
+    b->ang_t = ang_t;
+    return b->precision == T2_FP64 ? write_tables<double>(b)
+                                   : write_tables<float>(b);
+}
+
 // Level form: members grouped by group count; each group's schedules merged
 // step by step into one record stream with a problem table.
 template <typename T>
@@ -1642,12 +1719,7 @@ int build_level_group(t2_batch *b, LevelGroup &g)
             return fail(T2_ERR_HIP, std::string("batch merge: ") +
                                         hipGetErrorString(e));
         }
-        desc[j].nc = m->nc;
-        desc[j].sigma_mode = m->sigma_mode;
-        desc[j].sigma = static_cast<const T *>(m->d_sigt);
-        desc[j].src_iso = static_cast<const T *>(m->d_src);
-        desc[j].psi = static_cast<T *>(m->d_psi);
-        desc[j].lag_buf = static_cast<const T *>(m->d_lag_buf);
+        desc[j] = problem_desc<T>(m, b->ang_t[g.idx[j]]);
     }
     (void)hipFree(d_off);
     T2_HIP(hipMalloc(reinterpret_cast<void **>(&g.d_active),
@@ -1696,22 +1768,7 @@ int build_batch_queue(t2_batch *b)
             T2_HIP(hipStreamSynchronize(m->stream));
             T2_TRY(build_queue(m, 0));
         }
-        t2::BatchProblem<T> &bp = probs[i];
-        bp.a = t2::SweepArgs<T>{};
-        bp.a.geo = static_cast<const t2::ItemGeo<T> *>(m->d_item_geo);
-        bp.a.topo = m->d_item_topo;
-        bp.a.n_items = m->nc * m->na;
-        bp.a.nc = m->nc;
-        bp.a.sigma_mode = m->sigma_mode;
-        bp.a.sigma = static_cast<const T *>(m->d_sigt);
-        bp.a.src_iso = static_cast<const T *>(m->d_src);
-        bp.a.psi = static_cast<T *>(m->d_psi);
-        bp.a.lag_buf = static_cast<const T *>(m->d_lag_buf);
-        bp.pending = m->d_pending;
-        bp.down = m->d_down;
-        bp.arm = m->d_arm;
-        bp.first_item = m->queue_item;
-        bp.groups = int32_t(m->ng);
+        probs[i] = batch_problem<T>(m, b->ang_t[i]);
     }
     std::vector<t2::BatchChunk> chunks;
     for (int64_t st = 0; st < b->n_steps; st++)
@@ -1938,6 +1995,7 @@ int t2_batch_create(t2_batch **out, t2_sweeper *const *members, int count)
         hipSuccess)
         return batch_fail_cleanup(b, fail(T2_ERR_HIP, "event creation"));
     b->active.assign(count, 1);
+    b->ang_t.assign(count, nullptr);
     if (hipMalloc(reinterpret_cast<void **>(&b->d_active),
                   sizeof(int32_t) * count) != hipSuccess)
         return batch_fail_cleanup(b, fail(T2_ERR_NOMEM, "batch flags"));
@@ -2060,6 +2118,50 @@ static double seconds_since(std::chrono::steady_clock::time_point t0)
         .count();
 }
 
+// Advances the members' GMRES solves in lock step: every unfinished member
+// runs to its next operator apply y = (I - D L^-1 M S) v, one batched sweep
+// serves all the applies of a round, and a member whose GMRES needs no more
+// applies drops out of the sweeps.  done_at[i] (may be null) receives the
+// time from t0 at which member i finished.
+static int batch_gmres(t2_batch *b, std::vector<Gmres> &gm,
+                       std::chrono::steady_clock::time_point t0,
+                       double *done_at)
+{
+    const size_t nm = b->members.size();
+    std::vector<int32_t> act(nm, 0);
+    std::vector<bool> done(nm, false);
+    for (;;) {
+        bool any = false;
+        for (size_t i = 0; i < nm; i++) {
+            act[i] = 0;
+            if (done[i])
+                continue;
+            bool need = false;
+            T2_TRY(gm[i].next(&need));
+            if (need) {
+                act[i] = 1;
+                any = true;
+                T2_TRY(total_source(b->members[i], nullptr, gm[i].req_in));
+            } else {
+                done[i] = true;
+                if (done_at)
+                    done_at[i] = seconds_since(t0);
+            }
+        }
+        if (!any)
+            return T2_OK;
+        T2_TRY(batch_sweep_some(b, act));
+        for (size_t i = 0; i < nm; i++) {
+            if (!act[i])
+                continue;
+            t2_sweeper *m = b->members[i];
+            T2_TRY(psi_to_phi(m, gm[i].req_out));
+            T2_HIP(t2::launch_axpby(1.0, gm[i].req_in, -1.0, gm[i].req_out,
+                                    m->n_phi(), m->stream));
+        }
+    }
+}
+
 int t2_batch_solve_krylov(t2_batch *b, int64_t iter_max, double rtol,
                           int64_t restart, int64_t *n_iter, double *seconds)
 {
@@ -2070,10 +2172,6 @@ int t2_batch_solve_krylov(t2_batch *b, int64_t iter_max, double rtol,
     T2_HIP(hipSetDevice(b->device));
     const auto t0 = std::chrono::steady_clock::now();
     const size_t nm = b->members.size();
-    if (n_iter)
-        std::fill(n_iter, n_iter + nm, 0);
-    if (seconds)
-        std::fill(seconds, seconds + nm, 0.0);
     std::vector<double *> V(nm), rhs(nm);
     for (size_t i = 0; i < nm; i++) {
         t2_sweeper *m = b->members[i];
@@ -2092,48 +2190,160 @@ int t2_batch_solve_krylov(t2_batch *b, int64_t iter_max, double rtol,
         T2_TRY(gm[i].init(m, rhs[i], m->d_phi, restart, rtol, iter_max, V[i],
                           nullptr));
     }
-    // every member's GMRES advances to its next operator apply; one
-    // batched sweep serves all the applies of a round
-    std::vector<bool> done(nm, false);
-    for (;;) {
-        bool any = false;
-        for (size_t i = 0; i < nm; i++) {
-            act[i] = 0;
-            if (done[i])
-                continue;
-            bool need = false;
-            T2_TRY(gm[i].next(&need));
-            if (need) {
-                act[i] = 1;
-                any = true;
-                T2_TRY(total_source(b->members[i], nullptr, gm[i].req_in));
-            } else {
-                done[i] = true;
-                if (n_iter)
-                    n_iter[i] = gm[i].it;
-                if (seconds)
-                    seconds[i] = seconds_since(t0);
-            }
-        }
-        if (!any)
-            break;
-        T2_TRY(batch_sweep_some(b, act));
-        for (size_t i = 0; i < nm; i++) {
-            if (!act[i])
-                continue;
-            t2_sweeper *m = b->members[i];
-            T2_TRY(psi_to_phi(m, gm[i].req_out));
-            T2_HIP(t2::launch_axpby(1.0, gm[i].req_in, -1.0, gm[i].req_out,
-                                    m->n_phi(), m->stream));
-        }
+    std::vector<double> done_at(nm, 0.0);
+    T2_TRY(batch_gmres(b, gm, t0, done_at.data()));
+    for (size_t i = 0; i < nm; i++) {
+        if (n_iter)
+            n_iter[i] = gm[i].it;
+        if (seconds)
+            seconds[i] = done_at[i];
     }
     // psi = L^-1 (q + M S phi) for every member
     for (auto *m : b->members)
         T2_TRY(total_source(m, m->d_qiso, m->d_phi));
-    std::fill(act.begin(), act.end(), 1);
     T2_TRY(batch_sweep_some(b, act));
     for (auto *m : b->members)
         T2_HIP(hipStreamSynchronize(m->stream));
+    return batch_check_abort(b);
+}
+
+// The steps of the one-handle refined solve for every member, each stage in
+// lock step; per member the device work is the same sequence of operations,
+// so results are bitwise those of t2_solve_refined on the member alone.
+static int batch_refined_body(t2_batch *b, int64_t iter_max,
+                              double rtol_sweep, double rtol_correction,
+                              int64_t restart, t2_refine_info *info,
+                              double *seconds)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t nm = b->members.size();
+    std::vector<double *> V(nm), rhs(nm), y(nm);
+    std::vector<int64_t> sweeps0(nm);
+    for (size_t i = 0; i < nm; i++) {
+        t2_sweeper *m = b->members[i];
+        const int64_t n = m->n_phi();
+        double *work = nullptr;
+        T2_TRY(krylov_space(m, restart, 2, &V[i], &work));
+        rhs[i] = work;
+        y[i] = work + n;
+        if (!m->d_res)
+            T2_TRY(dalloc_t(m, &m->d_res, sizeof(float) * m->n_psi()));
+        if (!m->d_src64)
+            T2_TRY(dalloc_t(m, &m->d_src64, sizeof(double) * n));
+        sweeps0[i] = m->n_sweeps;
+        T2_TRY(zero_psi(m));
+        T2_TRY(total_source(m, m->d_qiso, nullptr));
+    }
+    std::vector<int32_t> all(nm, 1);
+    std::vector<t2_refine_info> st(nm);
+    std::vector<Gmres> gm(nm);
+
+    // 1. FP32-sweep GMRES on phi, then psi32 and phi1 = D psi32
+    T2_TRY(batch_sweep_some(b, all));
+    for (size_t i = 0; i < nm; i++) {
+        t2_sweeper *m = b->members[i];
+        T2_TRY(psi_to_phi(m, rhs[i]));
+        T2_TRY(gm[i].init(m, rhs[i], m->d_phi, restart, rtol_sweep, iter_max,
+                          V[i], nullptr));
+    }
+    T2_TRY(batch_gmres(b, gm, t0, nullptr));
+    for (size_t i = 0; i < nm; i++) {
+        t2_sweeper *m = b->members[i];
+        st[i].iters_sweep = gm[i].it;
+        st[i].residual_sweep = gm[i].rel_res();
+        T2_TRY(total_source(m, m->d_qiso, m->d_phi));
+    }
+    T2_TRY(batch_sweep_some(b, all));
+
+    // 2. FP64 defect r = q + M S phi1 - L psi32 of each member
+    std::vector<const void *> res(nm);
+    for (size_t i = 0; i < nm; i++) {
+        t2_sweeper *m = b->members[i];
+        double *phi_1 = m->d_phi_old;
+        T2_TRY(psi_to_phi(m, phi_1));
+        T2_HIP(t2::launch_total_source<double>(m->d_qiso, m->d_sigs, phi_1,
+                                               m->nc, int(m->ng), m->d_src64,
+                                               m->stream));
+        t2::ResidualArgs<float> ra{};
+        ra.geo = m->d_item_geo64;
+        ra.topo = m->d_item_topo;
+        ra.n_items = m->nc * m->na;
+        ra.nc = m->nc;
+        ra.sigma_mode = m->sigma_mode;
+        ra.sigma = m->d_sigt64;
+        ra.src_iso = m->d_src64;
+        ra.src_ang = nullptr;
+        ra.lag_up = m->d_lag_up;
+        ra.psi = static_cast<const float *>(m->d_psi);
+        ra.r = m->d_res;
+        T2_HIP(t2::launch_residual<float>(ra, int(m->ng), m->stream));
+        res[i] = m->d_res;
+        T2_TRY(total_source(m, nullptr, nullptr));
+    }
+
+    // 3. correction: GMRES on (I - D L^-1 M S) D delta = D L^-1 r
+    T2_TRY(batch_set_ang_t(b, res));
+    T2_TRY(batch_sweep_some(b, all));
+    T2_TRY(batch_set_ang_t(b, std::vector<const void *>(nm, nullptr)));
+    for (size_t i = 0; i < nm; i++) {
+        t2_sweeper *m = b->members[i];
+        T2_TRY(psi_to_phi(m, rhs[i]));
+        T2_TRY(gm[i].init(m, rhs[i], y[i], restart, rtol_correction,
+                          iter_max, V[i], nullptr));
+    }
+    std::vector<double> done_at(nm, 0.0);
+    T2_TRY(batch_gmres(b, gm, t0, done_at.data()));
+    for (size_t i = 0; i < nm; i++) {
+        t2_sweeper *m = b->members[i];
+        const int64_t n = m->n_phi();
+        st[i].iters_correction = gm[i].it;
+        st[i].residual_correction = gm[i].rel_res();
+        // y is D delta to the correction's tolerance: phi = phi1 + y
+        double corr = 0.0, base = 0.0;
+        T2_TRY(dot(m, y[i], y[i], n, &corr));
+        T2_HIP(t2::launch_axpby(1.0, m->d_phi_old, 1.0, y[i], n, m->stream));
+        T2_HIP(hipMemcpyAsync(m->d_phi, y[i], sizeof(double) * n,
+                              hipMemcpyDeviceToDevice, m->stream));
+        T2_TRY(dot(m, m->d_phi, m->d_phi, n, &base));
+        st[i].correction_rel = base > 0 ? std::sqrt(corr / base) : 0.0;
+        T2_TRY(total_source(m, m->d_qiso, m->d_phi));
+    }
+    // 4. psi consistent with the final phi (FP32 storage)
+    T2_TRY(batch_sweep_some(b, all));
+    for (size_t i = 0; i < nm; i++) {
+        t2_sweeper *m = b->members[i];
+        T2_HIP(hipStreamSynchronize(m->stream));
+        st[i].sweeps = m->n_sweeps - sweeps0[i];
+        if (info)
+            info[i] = st[i];
+        if (seconds)
+            seconds[i] = done_at[i];
+    }
+    return T2_OK;
+}
+
+int t2_batch_solve_refined(t2_batch *b, int64_t iter_max, double rtol_sweep,
+                           double rtol_correction, int64_t restart,
+                           t2_refine_info *info, double *seconds)
+{
+    t2::clear_error();
+    T2_TRY(batch_check(b));
+    if (b->precision != T2_FP32)
+        return fail(T2_ERR_STATE, "the refined solve needs FP32 members");
+    if (restart < 1 || restart > 60 || iter_max < 0)
+        return fail(T2_ERR_ARG, "restart must be in [1, 60]");
+    T2_HIP(hipSetDevice(b->device));
+    int rc = batch_refined_body(b, iter_max, rtol_sweep, rtol_correction,
+                                restart, info, seconds);
+    if (rc != T2_OK) {
+        // later sweeps must not see a residual source left in the tables
+        const std::string msg = t2_last_error();
+        (void)hipStreamSynchronize(b->stream);
+        b->ang_t.assign(b->members.size(), nullptr);
+        (void)(b->precision == T2_FP64 ? write_tables<double>(b)
+                                       : write_tables<float>(b));
+        return fail(rc, msg);
+    }
     return batch_check_abort(b);
 }
 

@@ -277,12 +277,14 @@ __device__ __forceinline__ void solve_item(const SweepArgs<T> &p,
     int sigma_mode = p.sigma_mode;
     const T *sigma = p.sigma, *src_iso = p.src_iso, *lag_buf = p.lag_buf;
     T *psi = p.psi;
+    const T *src_ang_t = p.src_ang_t;
     if (p.problems) {
         const ProblemDesc<T> &d = p.problems[t.problem];
         nc = d.nc;
         sigma_mode = d.sigma_mode;
         sigma = d.sigma;
         src_iso = d.src_iso;
+        src_ang_t = d.src_ang_t;
         lag_buf = d.lag_buf;
         psi = d.psi;
     }
@@ -319,8 +321,8 @@ __device__ __forceinline__ void solve_item(const SweepArgs<T> &p,
             for (int v = 0; v < 4; v++)
                 q[v] += T(qa[v * G]);
         }
-        if (p.src_ang_t) {
-            const T *qa = p.src_ang_t + size_t(t.angle) * plane + own + g;
+        if (src_ang_t) {
+            const T *qa = src_ang_t + size_t(t.angle) * plane + own + g;
 #pragma unroll
             for (int v = 0; v < 4; v++)
                 q[v] += qa[v * G];

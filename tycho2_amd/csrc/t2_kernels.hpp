@@ -51,6 +51,7 @@ struct ProblemDesc {
     int sigma_mode;
     const T *sigma;
     const T *src_iso;
+    const T *src_ang_t;            // [na][nc][4][G] in T, or null
     T *psi;
     const T *lag_buf;
 };

@@ -93,6 +93,9 @@ SIGNATURES = {
         _ip, _dp]),
     "t2_batch_solve_source_iteration": (ctypes.c_int, [
         ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, _ip, _dp]),
+    "t2_batch_solve_refined": (ctypes.c_int, [
+        ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+        ctypes.c_int64, ctypes.c_void_p, _dp]),
     "t2_batch_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "t2_batch_get_stream": (ctypes.c_int, [ctypes.c_void_p,
                                            ctypes.POINTER(ctypes.c_void_p)]),
@@ -444,6 +447,20 @@ class SweepBatch:
             self.handle, int(iter_max), float(err_max), _ptr(its, _ip),
             _ptr(sec)))
         return its, sec
+
+    def solve_refined(self, iter_max: int = 300, rtol_sweep: float = 1e-6,
+                      rtol_correction: float = 1e-5, restart: int = 20):
+        """Every FP32 member's refined solve (FP32 sweeps + FP64 defect
+        correction), stage by stage in lock step; returns (info dict per
+        member, seconds to the end of each member's correction)."""
+        n = len(self.members)
+        infos = (T2RefineInfo * n)()
+        sec = np.zeros(n)
+        _check(lib().t2_batch_solve_refined(
+            self.handle, int(iter_max), float(rtol_sweep),
+            float(rtol_correction), int(restart), infos, _ptr(sec)))
+        return [{k: getattr(x, k) for k, _ in T2RefineInfo._fields_}
+                for x in infos], sec
 
     def info(self) -> dict:
         n_steps, n_items = ctypes.c_int64(), ctypes.c_int64()
