@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define T2_ABI_VERSION 2
+#define T2_ABI_VERSION 3
 
 /* status codes */
 #define T2_OK 0
@@ -79,6 +79,9 @@ typedef struct {
     int32_t groups_per_lane;
     int32_t chunk_rounds;     /* queue knob (t2_set_engine)                */
     int32_t queue_first_step; /* first step run by the queue (-1: none)    */
+    int32_t intra_angle_p;    /* priority schedule rules (t2_set_priority; */
+    int32_t inter_angle_p;    /*   -1 and 0: level sets)                   */
+    int64_t max_cells_per_step;
 } t2_info;
 
 /* Message of the last failure on this thread ("" if none). */
@@ -104,6 +107,36 @@ int t2_sweep_levels(int64_t n_cells, const int64_t *adj_cell,
                     const double *area, const double *normal,
                     int64_t n_angles, const double *omega, int32_t *levels,
                     int64_t *n_lagged);
+
+/* Traversal priorities of the reference's graph traverser on one rank,
+ * priorities[a][c] (Priorities::calcPriorities, src/Priorities.cc:396-435;
+ * deck keys intraAngleP, interAngleP, src/Main.cc:111-112):
+ *   intra_angle_p  0 random (a seeded 31-bit draw per item), 1 b-level
+ *                  (longest downwind path), 2 BFDS, 3 DFDS, 4 DFHDS;
+ *   inter_angle_p  0 interleaved, 1 globally prioritised angles (higher
+ *                  index first), 2 locally prioritised angles.
+ * Unsigned 64-bit arithmetic as in the reference; with no partition sides
+ * on one rank, DFDS and DFHDS coincide.  Directions with lagged faces use
+ * the dependency graph left after lagging. */
+int t2_sweep_priorities(int64_t n_cells, const int64_t *adj_cell,
+                        const double *area, const double *normal,
+                        int64_t n_angles, const double *omega,
+                        int intra_angle_p, int inter_angle_p,
+                        uint64_t *priorities);
+
+/* The priority list schedule t2_set_priority runs: each step holds up to
+ * max_cells_per_step (0: no limit) of the (angle, cell) items ready when it
+ * starts, highest priority first (ties: lower angle, then lower cell) --
+ * GraphTraverser::traverse's ready heap popped maxCellsPerStep times per
+ * step (src/GraphTraverser.cc:518-536).  items[n_cells * n_angles] receives
+ * angle * n_cells + cell in schedule order, step_off[n_steps + 1] (room for
+ * n_cells * n_angles + 1) the step boundaries; order as t2_set_order. */
+int t2_priority_schedule(int64_t n_cells, const int64_t *adj_cell,
+                         const double *area, const double *normal,
+                         int64_t n_angles, const double *omega,
+                         int intra_angle_p, int inter_angle_p,
+                         int64_t max_cells_per_step, int order,
+                         int64_t *items, int64_t *step_off, int64_t *n_steps);
 
 /* ------------------------------------------------------------ setup
  * Replaces the reference's per-mesh set-up (TychoMesh, Quadrature,
@@ -143,6 +176,16 @@ int t2_set_schedule(t2_sweeper *s, int angle_batches);
 #define T2_ORDER_CELL 0
 #define T2_ORDER_FACES 1
 int t2_set_order(t2_sweeper *s, int order);
+
+/* Priority schedule in place of the level sets (deck keys intraAngleP,
+ * interAngleP, maxCellsPerStep; src/Main.cc:110-112): the list schedule of
+ * t2_priority_schedule.  Every step is a set of independent items, so each
+ * engine runs it; for the ready queue it is the ticket order, the
+ * reference's heap policy.  intra_angle_p = -1 returns to level sets
+ * (default).  angle_batches does not apply to a priority schedule.  Results
+ * do not depend on it. */
+int t2_set_priority(t2_sweeper *s, int intra_angle_p, int inter_angle_p,
+                    int64_t max_cells_per_step);
 
 /* Sweep engine (GraphTraverser::traverse, src/GraphTraverser.cc:467-759):
  *   T2_ENGINE_LEVELS  one kernel launch per schedule step, the whole sweep

@@ -167,3 +167,63 @@ def test_face_order_bitwise(engine):
             out.append((s.phi(), s.psi_angle(5)))
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("engine,intra,inter,width", [
+    (ENGINE_QUEUE, 3, 1, 100), (ENGINE_QUEUE, 1, 0, 0),
+    (ENGINE_QUEUE, 4, 2, 4096), (ENGINE_QUEUE, 0, 0, 512),
+    (ENGINE_LEVELS, 3, 1, 1000), (ENGINE_HYBRID, 1, 1, 2000),
+    (ENGINE_LEVELS, 2, 2, 0)])
+def test_priority_schedule_bitwise(engine, intra, inter, width):
+    """The reference's heap priorities as the schedule (for the ready queue,
+    its ticket order): results equal the level sets' bit for bit."""
+    print(f"priority engine={engine} intra={intra} inter={inter} "
+          f"width={width}", flush=True)
+    args = problem(5, 4, 16, seed=95)
+    ref = run_steps(args, FP64, ENGINE_LEVELS)
+    sm, conn, omega, wt, st, ss, q = args
+    with SweeperHIP(sm, conn, omega, wt, 16, st, ss, FP64) as s:
+        s.set_engine(engine, SPIN)
+        s.set_priority(intra, inter, width)
+        inf = s.info()
+        assert (inf["intra_angle_p"], inf["inter_angle_p"],
+                inf["max_cells_per_step"]) == (intra, inter, width)
+        s.set_source_isotropic(q)
+        for _ in range(3):
+            s.sweep_step()
+        assert np.array_equal(s.phi(), ref[0])
+        assert np.array_equal(s.psi_angle(omega.shape[0] - 1), ref[1])
+        s.set_priority(-1)
+        assert s.info()["intra_angle_p"] == -1
+
+
+@pytest.mark.parametrize("engine", [ENGINE_LEVELS, ENGINE_QUEUE])
+def test_priority_schedule_cyclic_and_fp32(engine):
+    """Lagged faces (cyclic mesh) and FP32 storage under a priority
+    schedule: the oracle's sweep, and the level sets bit for bit."""
+    print(f"priority cyclic engine={engine}", flush=True)
+    sm = meshmod.kuhn_cube(3, 0.2, seed=3)
+    conn = meshmod.tycho_connectivity(sm)
+    omega, wt = orc.quadrature(2)
+    rng = np.random.default_rng(9)
+    st = rng.uniform(1.0, 2.0, (sm.n_cells, 1))
+    src = rng.uniform(0.1, 1.0, (sm.n_cells, omega.shape[0], 4, 1))
+    prev = rng.uniform(0.0, 1.0, src.shape)
+    psi = prev.copy()
+    with SweeperHIP(sm, conn, omega, wt, 1, st, 0.5 * st, FP64) as s:
+        s.set_engine(engine, SPIN)
+        s.set_priority(3, 1, 64)
+        assert s.info()["n_lagged"] > 0
+        s.sweep(psi, src)
+    ref, _ = orc.sweep(orc.build_mesh(sm), omega, 1, st, src, psi=prev.copy())
+    assert np.abs(psi - ref).max() <= 1e-12 * np.abs(ref).max()
+    args = problem(4, 4, 4, seed=96)
+    ref32 = run_steps(args, FP32, ENGINE_LEVELS)
+    sm, conn, omega, wt, st, ss, q = args
+    with SweeperHIP(sm, conn, omega, wt, 4, st, ss, FP32) as s:
+        s.set_engine(engine, SPIN)
+        s.set_priority(1, 2, 300)
+        s.set_source_isotropic(q)
+        for _ in range(3):
+            s.sweep_step()
+        assert np.array_equal(s.phi(), ref32[0])

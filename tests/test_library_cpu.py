@@ -32,7 +32,7 @@ def test_every_declared_symbol_is_exported_and_bound():
     for name in names:
         assert hasattr(lib, name), name
         assert name in sweeper.SIGNATURES, name
-    assert lib.t2_abi_version() == 2
+    assert lib.t2_abi_version() == 3
 
 
 def test_host_geometry_bitwise_equal_to_oracle(cube208):

@@ -46,6 +46,8 @@ struct t2_sweeper {
     int64_t nc = 0, na = 0, ng = 0;
     int precision = T2_FP64, gauss = T2_GAUSS_NOPIVOT;
     int block = 256, gpl = 1, batches = 1, order = T2_ORDER_FACES;
+    int intra = -1, inter = 0;       // priority schedule (intra < 0: levels)
+    int64_t step_width = 0;
     bool use_graph = true;
     t2::HostMesh hm;                 // kept for schedule rebuilds
     std::vector<double> omega_h, sigma_t_h;
@@ -564,9 +566,13 @@ int build_schedule_device(t2_sweeper *s)
 {
     const int64_t nc = s->nc, na = s->na, ng = s->ng;
     free_schedule(s);
-    T2_TRY(t2::build_schedule(s->hm, na, s->omega_h.data(), s->batches,
-                              s->order,
-                              s->sched));
+    if (s->intra >= 0)
+        T2_TRY(t2::build_priority_schedule(s->hm, na, s->omega_h.data(),
+                                           s->intra, s->inter, s->step_width,
+                                           s->order, s->sched));
+    else
+        T2_TRY(t2::build_schedule(s->hm, na, s->omega_h.data(), s->batches,
+                                  s->order, s->sched));
     const size_t n_items = size_t(nc) * na;
     std::vector<double> geo(size_t(nc) * t2::kGeoStride, 0.0);
     std::vector<int4> adj(nc);
@@ -1027,6 +1033,9 @@ int t2_sweeper_info(const t2_sweeper *s, t2_info *info)
     info->sweeps = s->n_sweeps;
     info->engine = s->engine;
     info->angle_batches = s->batches;
+    info->intra_angle_p = s->intra;
+    info->inter_angle_p = s->inter;
+    info->max_cells_per_step = s->step_width;
     info->items_per_block = int32_t(s->block / (s->ng / s->gpl));
     info->groups_per_lane = s->gpl;
     info->chunk_rounds = s->chunk_rounds;
@@ -1077,6 +1086,30 @@ int t2_set_order(t2_sweeper *s, int order)
         return T2_OK;
     T2_HIP(hipStreamSynchronize(s->stream));
     s->order = order;
+    return build_schedule_device(s);
+}
+
+int t2_set_priority(t2_sweeper *s, int intra_angle_p, int inter_angle_p,
+                    int64_t max_cells_per_step)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (intra_angle_p < -1 || intra_angle_p > 4 || inter_angle_p < 0 ||
+        inter_angle_p > 2 || max_cells_per_step < 0)
+        return fail(T2_ERR_ARG, "intra_angle_p must be in [-1, 4], "
+                                "inter_angle_p in [0, 2] and "
+                                "max_cells_per_step >= 0");
+    if (intra_angle_p < 0) {
+        inter_angle_p = 0;
+        max_cells_per_step = 0;
+    }
+    if (intra_angle_p == s->intra && inter_angle_p == s->inter &&
+        max_cells_per_step == s->step_width)
+        return T2_OK;
+    T2_HIP(hipStreamSynchronize(s->stream));
+    s->intra = intra_angle_p;
+    s->inter = inter_angle_p;
+    s->step_width = max_cells_per_step;
     return build_schedule_device(s);
 }
 

@@ -80,4 +80,25 @@ std::vector<int32_t> angle_batches(int64_t na, const double *omega,
 int build_schedule(const HostMesh &m, int64_t na, const double *omega,
                    int batches, int order, Schedule &s);
 
+// Traversal priorities of the reference's graph traverser on one rank
+// (src/Priorities.cc:396-435): intra-angle rule 0 random, 1 b-level,
+// 2 BFDS, 3 DFDS, 4 DFHDS; inter-angle rule 0 interleaved, 1 globally
+// prioritised angles, 2 locally prioritised angles.  prio[a][c], uint64
+// with the reference's wrap-around arithmetic.  `levels` and `lagged` are
+// what build_levels returns for the same mesh and directions.
+int sweep_priorities(const HostMesh &m, int64_t na, const double *omega,
+                     const int32_t *levels,
+                     const std::vector<LaggedFace> &lagged, int intra,
+                     int inter, std::vector<uint64_t> &prio);
+
+// Items in priority list-schedule order: each step takes up to `width`
+// (0: all) of the items ready when it starts, highest priority first (ties:
+// lower angle, then lower cell), so every step is a set of mutually
+// independent items and every item comes after its upwind items.  Within a
+// step, items stay in that order (T2_ORDER_CELL) or are grouped stably by
+// their pattern of incoming interior faces (T2_ORDER_FACES).
+int build_priority_schedule(const HostMesh &m, int64_t na,
+                            const double *omega, int intra, int inter,
+                            int64_t width, int order, Schedule &s);
+
 }  // namespace t2

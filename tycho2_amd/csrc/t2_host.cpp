@@ -5,6 +5,7 @@
 #include <array>
 #include <atomic>
 #include <cmath>
+#include <random>
 #include <thread>
 
 #include "t2_common.hpp"
@@ -378,6 +379,298 @@ int build_schedule(const HostMesh &m, int64_t na, const double *omega,
     return T2_OK;
 }
 
+// ------------------------------------------------------------ priorities
+namespace {
+
+// The sweep dependency graph of every direction as the level sets resolve
+// it: (a, c) -> (a, u) when face k of c is outgoing (Omega.n > 0, the
+// reference's isOutgoing, src/TychoMesh.hh:107-108), u = adj[c][k], and
+// u's dependency through its face towards c is not lagged.
+struct SweepDag {
+    const HostMesh &m;
+    int64_t na;
+    const double *omega;
+    std::vector<uint8_t> lag;  // [na][nc]: bit k, face k's dependency lagged
+
+    SweepDag(const HostMesh &m_, int64_t na_, const double *omega_,
+             const std::vector<LaggedFace> &lagged)
+        : m(m_), na(na_), omega(omega_), lag(size_t(na_) * m_.nc, 0)
+    {
+        for (const LaggedFace &f : lagged)
+            lag[size_t(f.angle) * m.nc + f.cell] |= uint8_t(1u << f.face);
+    }
+
+    bool is_dep(int64_t a, int64_t c, int k) const
+    {
+        return m.adj[4 * c + k] >= 0 &&
+               !(omega_dot_n(omega + 3 * a, &m.normal[(4 * c + k) * 3]) > 0) &&
+               !(lag[size_t(a) * m.nc + c] >> k & 1u);
+    }
+
+    int down(int64_t a, int64_t c, int64_t out[4]) const
+    {
+        int n = 0;
+        for (int k = 0; k < 4; k++) {
+            const int64_t u = m.adj[4 * c + k];
+            if (u < 0 ||
+                !(omega_dot_n(omega + 3 * a, &m.normal[(4 * c + k) * 3]) > 0))
+                continue;
+            int back = 0;
+            while (back < 4 && m.adj[4 * u + back] != c)
+                back++;
+            if (back < 4 && is_dep(a, u, back))
+                out[n++] = u;
+        }
+        return n;
+    }
+
+    int in_degree(int64_t a, int64_t c) const
+    {
+        int n = 0;
+        for (int k = 0; k < 4; k++)
+            n += is_dep(a, c, k);
+        return n;
+    }
+};
+
+// Cells of one angle in decreasing level order (a reverse topological order
+// of its dependency graph).
+void reverse_level_order(const int32_t *level, int64_t nc,
+                         std::vector<int32_t> &cells)
+{
+    int32_t top = 0;
+    for (int64_t c = 0; c < nc; c++)
+        top = std::max(top, level[c]);
+    std::vector<int64_t> start(size_t(top) + 2, 0);
+    for (int64_t c = 0; c < nc; c++)
+        start[size_t(top - level[c]) + 1]++;
+    for (int32_t l = 0; l <= top; l++)
+        start[l + 1] += start[l];
+    cells.resize(nc);
+    for (int64_t c = 0; c < nc; c++)
+        cells[start[size_t(top - level[c])]++] = int32_t(c);
+}
+
+}  // namespace
+
+int sweep_priorities(const HostMesh &m, int64_t na, const double *omega,
+                     const int32_t *levels,
+                     const std::vector<LaggedFace> &lagged, int intra,
+                     int inter, std::vector<uint64_t> &prio)
+{
+    if (intra < 0 || intra > 4 || inter < 0 || inter > 2)
+        return fail(T2_ERR_ARG, "intraAngleP must be in [0, 4] and "
+                                "interAngleP in [0, 2]");
+    const int64_t nc = m.nc;
+    const SweepDag dag(m, na, omega, lagged);
+    std::vector<uint64_t> blev;
+    try {
+        prio.assign(size_t(na) * nc, 0);
+        blev.assign(size_t(na) * nc, 0);
+    } catch (...) {
+        return fail(T2_ERR_NOMEM, "host memory for sweep priorities");
+    }
+    // b-level: longest path to a cell with no downwind dependant, by the
+    // backward traversal of src/Priorities.cc:120-143 (one rank: no
+    // partition sides)
+    std::vector<int32_t> cells;
+    uint64_t max_blevel = 0;
+    for (int64_t a = 0; a < na; a++) {
+        reverse_level_order(levels + a * nc, nc, cells);
+        uint64_t *b = blev.data() + a * nc;
+        for (int32_t c : cells) {
+            int64_t d[4];
+            const int n = dag.down(a, c, d);
+            uint64_t v = 0;
+            for (int i = 0; i < n; i++)
+                v = std::max(v, b[d[i]] + 1);
+            b[c] = v;
+            max_blevel = std::max(max_blevel, v);
+        }
+    }
+    switch (intra) {
+    case 0: {  // random (src/Priorities.cc:283-291), a seeded 31-bit draw
+        std::mt19937 gen(0);
+        for (int64_t a = 0; a < na; a++)
+            for (int64_t c = 0; c < nc; c++)
+                prio[size_t(a) * nc + c] = uint64_t(gen() >> 1);
+        break;
+    }
+    case 1:  // b-levels (src/Priorities.cc:297-305)
+        prio = blev;
+        break;
+    default: {
+        // BFDS / DFDS / DFHDS (src/Priorities.cc:170-262, 311-322): on one
+        // rank no face is a partition side, so only the parent shift (0 for
+        // BFDS, -1 otherwise, added in unsigned 64-bit arithmetic) acts
+        const uint64_t shift = intra == 2 ? 0 : ~uint64_t(0);
+        for (int64_t a = 0; a < na; a++) {
+            reverse_level_order(levels + a * nc, nc, cells);
+            uint64_t *p = prio.data() + a * nc;
+            for (int32_t c : cells) {
+                int64_t d[4];
+                const int n = dag.down(a, c, d);
+                uint64_t v = 0;
+                for (int i = 0; i < n; i++)
+                    v = std::max(v, p[d[i]] + shift);
+                p[c] = v;
+            }
+        }
+        break;
+    }
+    }
+    // inter-angle rule (src/Priorities.cc:328-375); nlevels is the largest
+    // b-level (src/Priorities.cc:400-401)
+    const uint64_t span = max_blevel * uint64_t(nc);
+    if (inter == 1) {
+        for (int64_t a = 0; a < na; a++)
+            for (int64_t c = 0; c < nc; c++)
+                prio[size_t(a) * nc + c] += uint64_t(a) * span;
+    } else if (inter == 2) {
+        std::vector<std::pair<uint64_t, uint64_t>> high(na);
+        for (int64_t a = 0; a < na; a++) {
+            uint64_t h = 0;
+            for (int64_t c = 0; c < nc; c++)
+                h = std::max(h, prio[size_t(a) * nc + c]);
+            high[a] = {h, uint64_t(a)};
+        }
+        std::sort(high.begin(), high.end());
+        uint64_t rank = 0;
+        for (auto it = high.rbegin(); it != high.rend(); ++it) {
+            ++rank;
+            const int64_t a = int64_t(it->second);
+            for (int64_t c = 0; c < nc; c++)
+                prio[size_t(a) * nc + c] += (uint64_t(na) - rank) * span;
+        }
+    }
+    return T2_OK;
+}
+
+int build_priority_schedule(const HostMesh &m, int64_t na,
+                            const double *omega, int intra, int inter,
+                            int64_t width, int order, Schedule &s)
+{
+    const int64_t nc = m.nc;
+    int cb = 1;
+    while ((int64_t(1) << cb) < nc)
+        cb++;
+    int ab = 1;
+    while ((int64_t(1) << ab) < na)
+        ab++;
+    if (cb + ab > 32)
+        return fail(T2_ERR_ARG, "cells x angles exceed the 32-bit work item");
+    if (width < 0)
+        return fail(T2_ERR_ARG, "max cells per step must be >= 0");
+    s.cell_bits = cb;
+    std::vector<int32_t> levels;
+    try {
+        levels.resize(size_t(na) * nc);
+    } catch (...) {
+        return fail(T2_ERR_NOMEM, "host memory for level sets");
+    }
+    s.lagged.clear();
+    int rc = build_levels(m, na, omega, levels.data(), s.lagged);
+    if (rc)
+        return rc;
+    int32_t min_lev = INT32_MAX;
+    for (int64_t a = 0; a < na; a++)
+        min_lev = std::min(min_lev,
+                           1 + *std::max_element(levels.begin() + a * nc,
+                                                 levels.begin() + (a + 1) * nc));
+    s.min_levels = min_lev;
+    std::vector<uint64_t> prio;
+    rc = sweep_priorities(m, na, omega, levels.data(), s.lagged, intra, inter,
+                          prio);
+    if (rc)
+        return rc;
+    levels = std::vector<int32_t>();
+    const SweepDag dag(m, na, omega, s.lagged);
+    struct Ready {
+        uint64_t p;
+        int32_t a, c;
+    };
+    // max-heap: highest priority, then lowest angle, then lowest cell
+    auto below = [](const Ready &x, const Ready &y) {
+        if (x.p != y.p)
+            return x.p < y.p;
+        if (x.a != y.a)
+            return x.a > y.a;
+        return x.c > y.c;
+    };
+    std::vector<uint8_t> indeg;
+    std::vector<Ready> heap;
+    try {
+        indeg.resize(size_t(na) * nc);
+        s.work.clear();
+        s.work.reserve(size_t(na) * nc);
+    } catch (...) {
+        return fail(T2_ERR_NOMEM, "host memory for the priority schedule");
+    }
+    for (int64_t a = 0; a < na; a++)
+        for (int64_t c = 0; c < nc; c++) {
+            const int d = dag.in_degree(a, c);
+            indeg[size_t(a) * nc + c] = uint8_t(d);
+            if (d == 0)
+                heap.push_back({prio[size_t(a) * nc + c], int32_t(a),
+                                int32_t(c)});
+        }
+    std::make_heap(heap.begin(), heap.end(), below);
+    s.step_off.assign(1, 0);
+    std::vector<Ready> step;
+    std::vector<uint32_t> sorted;
+    while (!heap.empty()) {
+        step.clear();
+        while (!heap.empty() && (width == 0 || int64_t(step.size()) < width)) {
+            std::pop_heap(heap.begin(), heap.end(), below);
+            step.push_back(heap.back());
+            heap.pop_back();
+        }
+        const size_t first = s.work.size();
+        for (const Ready &r : step)
+            s.work.push_back(uint32_t(r.c) | (uint32_t(r.a) << cb));
+        if (order == T2_ORDER_FACES) {
+            // stable grouping by incoming-face pattern within the step
+            std::array<size_t, 17> start{};
+            std::vector<uint8_t> mask(step.size());
+            for (size_t i = 0; i < step.size(); i++) {
+                unsigned mk = 0;
+                for (int k = 0; k < 4; k++)
+                    if (m.adj[4 * int64_t(step[i].c) + k] >= 0 &&
+                        omega_dot_n(omega + 3 * int64_t(step[i].a),
+                                    &m.normal[(4 * int64_t(step[i].c) + k) *
+                                              3]) < 0)
+                        mk |= 1u << k;
+                mask[i] = uint8_t(mk);
+                start[mk + 1]++;
+            }
+            for (int b = 0; b < 16; b++)
+                start[b + 1] += start[b];
+            sorted.resize(step.size());
+            for (size_t i = 0; i < step.size(); i++)
+                sorted[start[mask[i]]++] = s.work[first + i];
+            std::copy(sorted.begin(), sorted.end(), s.work.begin() + first);
+        }
+        s.step_off.push_back(int64_t(s.work.size()));
+        for (const Ready &r : step) {
+            int64_t d[4];
+            const int n = dag.down(r.a, r.c, d);
+            for (int i = 0; i < n; i++) {
+                uint8_t &k = indeg[size_t(r.a) * nc + d[i]];
+                if (--k == 0) {
+                    heap.push_back({prio[size_t(r.a) * nc + d[i]], r.a,
+                                    int32_t(d[i])});
+                    std::push_heap(heap.begin(), heap.end(), below);
+                }
+            }
+        }
+    }
+    if (int64_t(s.work.size()) != na * nc)
+        return fail(T2_ERR_ARG, "priority schedule: dependency graph left "
+                                "items unscheduled");
+    s.n_steps = int64_t(s.step_off.size()) - 1;
+    return T2_OK;
+}
+
 }  // namespace t2
 
 // ---------------------------------------------------------------- C ABI
@@ -418,6 +711,70 @@ int t2_sweep_levels(int64_t n_cells, const int64_t *adj_cell,
     if (n_lagged)
         *n_lagged = int64_t(lagged.size());
     return rc;
+}
+
+
+int t2_sweep_priorities(int64_t n_cells, const int64_t *adj_cell,
+                        const double *area, const double *normal,
+                        int64_t n_angles, const double *omega,
+                        int intra_angle_p, int inter_angle_p,
+                        uint64_t *priorities)
+{
+    t2::clear_error();
+    if (n_cells <= 0 || n_angles <= 0 || !adj_cell || !area || !normal ||
+        !omega || !priorities)
+        return t2::fail(T2_ERR_ARG, "t2_sweep_priorities: bad argument");
+    t2::HostMesh m;
+    m.nc = n_cells;
+    m.adj.assign(adj_cell, adj_cell + 4 * n_cells);
+    m.area.assign(area, area + 4 * n_cells);
+    m.normal.assign(normal, normal + 12 * n_cells);
+    std::vector<int32_t> levels(size_t(n_angles) * n_cells);
+    std::vector<t2::LaggedFace> lagged;
+    int rc = t2::build_levels(m, n_angles, omega, levels.data(), lagged);
+    if (rc)
+        return rc;
+    std::vector<uint64_t> prio;
+    rc = t2::sweep_priorities(m, n_angles, omega, levels.data(), lagged,
+                              intra_angle_p, inter_angle_p, prio);
+    if (rc)
+        return rc;
+    std::copy(prio.begin(), prio.end(), priorities);
+    return T2_OK;
+}
+
+int t2_priority_schedule(int64_t n_cells, const int64_t *adj_cell,
+                         const double *area, const double *normal,
+                         int64_t n_angles, const double *omega,
+                         int intra_angle_p, int inter_angle_p,
+                         int64_t max_cells_per_step, int order,
+                         int64_t *items, int64_t *step_off, int64_t *n_steps)
+{
+    t2::clear_error();
+    if (n_cells <= 0 || n_angles <= 0 || !adj_cell || !area || !normal ||
+        !omega || !items || !step_off || !n_steps)
+        return t2::fail(T2_ERR_ARG, "t2_priority_schedule: bad argument");
+    if (order != T2_ORDER_CELL && order != T2_ORDER_FACES)
+        return t2::fail(T2_ERR_ARG, "order must be T2_ORDER_CELL or "
+                                    "T2_ORDER_FACES");
+    t2::HostMesh m;
+    m.nc = n_cells;
+    m.adj.assign(adj_cell, adj_cell + 4 * n_cells);
+    m.area.assign(area, area + 4 * n_cells);
+    m.normal.assign(normal, normal + 12 * n_cells);
+    t2::Schedule s;
+    int rc = t2::build_priority_schedule(m, n_angles, omega, intra_angle_p,
+                                         inter_angle_p, max_cells_per_step,
+                                         order, s);
+    if (rc)
+        return rc;
+    const uint32_t cmask = (uint32_t(1) << s.cell_bits) - 1;
+    for (size_t i = 0; i < s.work.size(); i++)
+        items[i] = int64_t(s.work[i] >> s.cell_bits) * n_cells +
+                   int64_t(s.work[i] & cmask);
+    std::copy(s.step_off.begin(), s.step_off.end(), step_off);
+    *n_steps = s.n_steps;
+    return T2_OK;
 }
 
 }  // extern "C"

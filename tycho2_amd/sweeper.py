@@ -34,6 +34,7 @@ ERR_TIMEOUT = -7
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int64)
 _i32p = ctypes.POINTER(ctypes.c_int32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
 _LIB = None
 
 # Every exported symbol of include/tycho2_sweep.h with its argument types.
@@ -44,6 +45,12 @@ SIGNATURES = {
                                         _ip, _ip, _dp, _dp, _dp]),
     "t2_sweep_levels": (ctypes.c_int, [ctypes.c_int64, _ip, _dp, _dp,
                                        ctypes.c_int64, _dp, _i32p, _ip]),
+    "t2_sweep_priorities": (ctypes.c_int, [ctypes.c_int64, _ip, _dp, _dp,
+                                           ctypes.c_int64, _dp, ctypes.c_int,
+                                           ctypes.c_int, _u64p]),
+    "t2_priority_schedule": (ctypes.c_int, [
+        ctypes.c_int64, _ip, _dp, _dp, ctypes.c_int64, _dp, ctypes.c_int,
+        ctypes.c_int, ctypes.c_int64, ctypes.c_int, _ip, _ip, _ip]),
     "t2_sweeper_create": (ctypes.c_int, [
         ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int64,
         ctypes.c_int64, _dp, _ip, _ip, _ip, ctypes.c_int64, _dp, _dp,
@@ -54,6 +61,8 @@ SIGNATURES = {
                                      ctypes.c_int]),
     "t2_set_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "t2_set_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "t2_set_priority": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int64]),
     "t2_set_engine": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int64]),
     "t2_set_queue_split": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
@@ -114,7 +123,10 @@ class T2Info(ctypes.Structure):
                 ("items_per_block", ctypes.c_int32),
                 ("groups_per_lane", ctypes.c_int32),
                 ("chunk_rounds", ctypes.c_int32),
-                ("queue_first_step", ctypes.c_int32)]
+                ("queue_first_step", ctypes.c_int32),
+                ("intra_angle_p", ctypes.c_int32),
+                ("inter_angle_p", ctypes.c_int32),
+                ("max_cells_per_step", ctypes.c_int64)]
 
 
 class T2RefineInfo(ctypes.Structure):
@@ -196,6 +208,34 @@ def sweep_levels(adj, area, normal, omega):
     return levels, lag.value
 
 
+def sweep_priorities(adj, area, normal, omega, intra: int, inter: int):
+    """The reference's traversal priorities prio[a][c] (uint64) on one rank
+    for deck values intraAngleP / interAngleP."""
+    adj, ar, nrm, om = _i64(adj), _f64(area), _f64(normal), _f64(omega)
+    nc, na = adj.shape[0], om.shape[0]
+    prio = np.empty((na, nc), dtype=np.uint64)
+    _check(lib().t2_sweep_priorities(nc, _ptr(adj, _ip), _ptr(ar), _ptr(nrm),
+                                     na, _ptr(om), int(intra), int(inter),
+                                     _ptr(prio, _u64p)))
+    return prio
+
+
+def priority_schedule(adj, area, normal, omega, intra: int, inter: int,
+                      max_cells_per_step: int, order: int = ORDER_CELL):
+    """(items, step_off): the priority list schedule, items as
+    angle * n_cells + cell in schedule order, steps items[off[i]:off[i+1]]."""
+    adj, ar, nrm, om = _i64(adj), _f64(area), _f64(normal), _f64(omega)
+    nc, na = adj.shape[0], om.shape[0]
+    items = np.empty(nc * na, dtype=np.int64)
+    off = np.empty(nc * na + 1, dtype=np.int64)
+    n = ctypes.c_int64(0)
+    _check(lib().t2_priority_schedule(
+        nc, _ptr(adj, _ip), _ptr(ar), _ptr(nrm), na, _ptr(om), int(intra),
+        int(inter), int(max_cells_per_step), int(order), _ptr(items, _ip),
+        _ptr(off, _ip), ctypes.byref(n)))
+    return items, off[:n.value + 1].copy()
+
+
 class SweeperHIP:
     """Device sweeper for one mesh, quadrature and cross-section set."""
 
@@ -263,6 +303,15 @@ class SweeperHIP:
     def set_order(self, order: int) -> None:
         """ORDER_CELL or ORDER_FACES: item order within a step."""
         _check(lib().t2_set_order(self.handle, int(order)))
+
+    def set_priority(self, intra_angle_p: int, inter_angle_p: int = 0,
+                     max_cells_per_step: int = 0) -> None:
+        """Priority list schedule (deck keys intraAngleP, interAngleP,
+        maxCellsPerStep) in place of the level sets; intra_angle_p = -1
+        returns to level sets."""
+        _check(lib().t2_set_priority(self.handle, int(intra_angle_p),
+                                     int(inter_angle_p),
+                                     int(max_cells_per_step)))
 
     def set_engine(self, engine: int, spin_limit: int = 0,
                    chunk_rounds: int = 0) -> None:
