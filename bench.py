@@ -176,7 +176,7 @@ def batch_leg(args, ev):
     import numpy as np
     from tycho2_amd import problems
     from tycho2_amd.sweeper import (ENGINE_LEVELS, ENGINE_QUEUE, FP64,
-                                    SweepBatch, SweeperHIP)
+                                    SweepBatch, SweeperHIP, T2Error)
     from tycho2_amd.timing import time_region
     t0 = time.time()
     ps = problems.batch(count=args.batch_count)
@@ -192,7 +192,13 @@ def batch_leg(args, ev):
     waves.append(cur)
     out = dict(problems=len(ps), waves=[], per_problem=[])
     tot = dict(levels=0.0, queue=0.0, best=0.0, solves=0, solve_s=0.0)
+    rule = (tuple(int(x) for x in args.batch_priority.split(","))
+            if args.batch_priority else None)
+    only = ({int(x) for x in args.batch_waves.split(",")}
+            if args.batch_waves else None)
     for w, wave in enumerate(waves):
+        if only is not None and w not in only:
+            continue
         t1 = time.time()
         hs = []
         for p in wave:
@@ -203,37 +209,62 @@ def batch_leg(args, ev):
             f"{time.time() - t1:.0f}s")
         solves = sum(p.solves_per_sweep for p in wave)
         rec = dict(problems=len(wave), solves_per_sweep=solves)
+        def timed(b, name, engine, rounds):
+            b.set_engine(engine, 0, rounds)
+            try:
+                b.sweep_only()
+            except T2Error as e:
+                # the level form refuses schedules of too many steps
+                log(f"wave {w}: {name} not run: {e}")
+                return float("inf")
+            b.synchronize()
+            ms = time_region(ev, b, b.sweep_only, args.steps) / args.steps
+            log(f"wave {w}: {name} {ms:.2f} ms/sweep, "
+                f"{solves / ms / 1e6:.1f} G solves/s")
+            return ms
+
+        timings = {}
         with SweepBatch(hs) as b:
-            timings = {}
             for name, engine, rounds in (("levels", ENGINE_LEVELS, 0),
                                          ("queue_r2", ENGINE_QUEUE, 2),
                                          ("queue_r4", ENGINE_QUEUE, 4),
                                          ("queue_r8", ENGINE_QUEUE, 8)):
-                b.set_engine(engine, 0, rounds)
-                b.sweep_only()
-                b.synchronize()
-                ms = time_region(ev, b, b.sweep_only, args.steps) / args.steps
-                timings[name] = ms
-                log(f"wave {w}: {name} {ms:.2f} ms/sweep, "
-                    f"{solves / ms / 1e6:.1f} G solves/s")
-            best_q = min((k for k in timings if k.startswith("queue")),
-                         key=timings.get)
-            rec["ms_per_sweep"] = {k: round(v, 3) for k, v in timings.items()}
-            rec["engine"] = ("queue" if timings[best_q] < timings["levels"]
-                             else "levels")
-            tot["levels"] += timings["levels"]
-            tot["queue"] += timings[best_q]
-            tot["best"] += min(timings["levels"], timings[best_q])
-            tot["solves"] += solves
-            b.set_engine(ENGINE_QUEUE if timings[best_q] < timings["levels"]
-                         else ENGINE_LEVELS, 0,
-                         int(best_q[len("queue_r"):]))
+                timings[name] = timed(b, name, engine, rounds)
+        if rule:
+            # the ready queue in the order of the reference's traversal
+            # priorities (members re-scheduled, so a new batch)
+            for h in hs:
+                h.set_priority(*rule)
+            with SweepBatch(hs) as b:
+                for name, rounds in (("queue_prio_r2", 2),
+                                     ("queue_prio_r4", 4)):
+                    timings[name] = timed(b, name, ENGINE_QUEUE, rounds)
+        best = min(timings, key=timings.get)
+        if rule and not best.startswith("queue_prio"):
+            for h in hs:
+                h.set_priority(-1)
+        rec["ms_per_sweep"] = {k: (round(v, 3) if v != float("inf")
+                                   else None)
+                               for k, v in timings.items()}
+        rec["engine"] = best
+        tot["levels"] += timings["levels"]
+        tot["queue"] += min(timings[k] for k in timings
+                            if k.startswith("queue_r"))
+        if rule:
+            tot["queue_prio"] = tot.get("queue_prio", 0.0) + min(
+                timings[k] for k in timings if k.startswith("queue_prio"))
+        tot["best"] += timings[best]
+        tot["solves"] += solves
+        with SweepBatch(hs) as b:
+            b.set_engine(ENGINE_LEVELS if best == "levels" else ENGINE_QUEUE,
+                         0, 0 if best == "levels" else int(best[-1]))
             t2 = time.time()
             its, sec = b.solve_krylov(300, 1e-10, 20)
             rec["solve_s"] = round(time.time() - t2, 3)
             tot["solve_s"] += rec["solve_s"]
-            log(f"wave {w}: batched GMRES(20) to 1e-10 in {rec['solve_s']}s, "
-                f"iterations {int(its.min())}-{int(its.max())}")
+            log(f"wave {w}: batched GMRES(20) to 1e-10 on {best} in "
+                f"{rec['solve_s']}s, iterations {int(its.min())}-"
+                f"{int(its.max())}")
         for p, h, k, t in zip(wave, hs, its, sec):
             out["per_problem"].append(dict(
                 name=p.name, cells=p.nc, angles=p.na, groups=p.ng,
@@ -249,15 +280,71 @@ def batch_leg(args, ev):
            "higher_is_better": True, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"{len(ps)} seeded Kuhn cubes, n in "
                                   "[10, 45], S4/6/8/12, 1/4/16/32 groups",
-                      "waves": len(waves)},
+                      "waves": len(waves),
+                      "waves_run": sorted(only) if only is not None
+                      else list(range(len(waves))),
+                      "schedule": ("level sets" if rule is None else
+                                   "level sets, and for the queue also "
+                                   "priorities intraAngleP/interAngleP/"
+                                   f"maxCellsPerStep = {list(rule)}")},
            "levels_solves_per_s": round(tot["solves"] /
                                         (tot["levels"] * 1e-3)),
            "queue_solves_per_s": round(tot["solves"] / (tot["queue"] * 1e-3)),
+           "queue_priority_solves_per_s": (
+               round(tot["solves"] / (tot["queue_prio"] * 1e-3))
+               if rule else None),
            "solve_wall_s": round(tot["solve_s"], 3),
            "time_to_solution_s": {"min": float(tts.min()),
                                   "median": float(np.median(tts)),
                                   "max": float(tts.max())},
            "detail": out}
+    print(json.dumps(res), flush=True)
+
+
+def schedule_leg(p, args, ev):
+    """Sweep time of the level sets against the reference's traversal
+    priorities (the ready queue's ticket order), per engine, on one FP64
+    handle at the default launch shape."""
+    from tycho2_amd.sweeper import (ENGINE_HYBRID, ENGINE_LEVELS,
+                                    ENGINE_QUEUE, FP64, SweeperHIP)
+    from tycho2_amd.timing import time_region
+    widths = [int(w) for w in args.study_widths.split(",")]
+    rules = [(-1, 0, 0)] + [(i, j, w) for w in widths
+                            for (i, j) in ((1, 0), (3, 1), (4, 2), (0, 0))]
+    rows = []
+    with SweeperHIP.from_problem(p, FP64) as s:
+        for rule in rules:
+            t0 = time.time()
+            s.set_priority(*rule)
+            build_s = round(time.time() - t0, 2)
+            n_steps = s.info()["n_steps"]
+            engines = ((ENGINE_LEVELS, ENGINE_QUEUE, ENGINE_HYBRID)
+                       if rule[0] < 0 else (ENGINE_QUEUE, ENGINE_LEVELS))
+            for engine in engines:
+                s.set_engine(engine)
+                if engine == ENGINE_LEVELS and n_steps > 20000:
+                    continue    # one launch per step: not a useful pairing
+                s.sweep_only()
+                s.synchronize()
+                ms = time_region(ev, s, s.sweep_only, args.steps) / args.steps
+                row = dict(intra=rule[0], inter=rule[1], width=rule[2],
+                           engine=engine, steps=n_steps, build_s=build_s,
+                           ms_per_sweep=round(ms, 3),
+                           solves_per_s=round(p.solves_per_sweep /
+                                              (ms * 1e-3)))
+                rows.append(row)
+                log(f"schedule study {row}")
+    base = {r["engine"]: r["ms_per_sweep"] for r in rows if r["intra"] < 0}
+    best = min(rows, key=lambda r: r["ms_per_sweep"])
+    res = {"metric": "DG cell-angle-group solves/s by sweep schedule "
+                     "(level sets vs traversal priorities), FP64 sweep",
+           "value": best["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+           "steps": args.steps, "higher_is_better": True, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": f"{args.config}: {p.nc} tets, {p.na} "
+                                  f"angles, {p.ng} groups",
+                      "solves_per_sweep": p.solves_per_sweep},
+           "level_sets_ms_per_sweep": base, "best": best, "rows": rows}
     print(json.dumps(res), flush=True)
 
 
@@ -285,12 +372,23 @@ def main() -> int:
     ap.add_argument("--batch-count", type=int, default=48)
     ap.add_argument("--batch-gb", type=float, default=200.0,
                     help="device memory per wave of the batch workload")
+    ap.add_argument("--batch-priority", default="",
+                    help="intraAngleP,interAngleP,maxCellsPerStep for every "
+                         "batch member (default: level sets)")
+    ap.add_argument("--batch-waves", default="",
+                    help="comma-separated waves of the batch workload to run "
+                         "(default: all)")
     ap.add_argument("--cpu-angles", type=int, default=0,
                     help="angles in the CPU-baseline sample (0: 2 x cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the two solves to 1e-10 (steps only)")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--schedule-study", action="store_true",
+                    help="only time level sets against traversal-priority "
+                         "schedules on each engine")
+    ap.add_argument("--study-widths", default="32768,262144",
+                    help="maxCellsPerStep values of the schedule study")
     ap.add_argument("--counter-run", choices=["fp64", "fp32"],
                     help="only a short run for rocprofv3 --pmc")
     args = ap.parse_args()
@@ -317,6 +415,9 @@ def main() -> int:
                     max(1, args.steps))
         return 0
     ev = HipEvents()
+    if args.schedule_study:
+        schedule_leg(p, args, ev)
+        return 0
     solve = not args.no_solve
     tune = not args.no_tune
     r64 = path_leg(p, FP64, args.steps, args.warmup, ev, tune, solve)

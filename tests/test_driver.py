@@ -15,6 +15,8 @@ snOrder         8
 iterMax         100
 errMax          1e-10
 maxCellsPerStep 100
+intraAngleP     3
+interAngleP     1
 nGroups         2
 sigmaT1         10
 sigmaS1         5
@@ -50,6 +52,8 @@ def test_driver_usage_and_errors(tmp_path):
     assert r.returncode == 1 and "snOrder" in r.stderr
     r = run(GOLD_DECK.format(out=out, extra="SweepEngine Fast"), tmp_path)
     assert r.returncode == 1 and "SweepEngine" in r.stderr
+    r = run(GOLD_DECK.format(out=out, extra="SweepSchedule Heap"), tmp_path)
+    assert r.returncode == 1 and "SweepSchedule" in r.stderr
     r = run(GOLD_DECK.format(out=out, extra="").replace(
         "GaussElim       NoPivot", "GaussElim       Fastest"), tmp_path)
     assert r.returncode == 1 and "GaussElim" in r.stderr
@@ -60,7 +64,10 @@ def test_driver_usage_and_errors(tmp_path):
     ("DeviceLoop false", "NoPivot"), ("DeviceLoop true", "NoPivot"),
     ("DeviceLoop true\nSweepEngine Queue", "NoPivot"),
     ("DeviceLoop true", "Original"), ("DeviceLoop true", "CramerGlu"),
-    ("DeviceLoop false", "CramerIntel")])
+    ("DeviceLoop false", "CramerIntel"),
+    ("DeviceLoop false\nSweepSchedule Priorities", "NoPivot"),
+    ("DeviceLoop true\nSweepEngine Queue\nSweepSchedule Priorities",
+     "NoPivot")])
 def test_driver_reproduces_gold(tmp_path, gold, extra, gauss):
     print(f"driver gold {extra} GaussElim {gauss}", flush=True)
     out = str(tmp_path / "out.psi")

@@ -227,3 +227,34 @@ def test_priority_schedule_cyclic_and_fp32(engine):
         for _ in range(3):
             s.sweep_step()
         assert np.array_equal(s.phi(), ref32[0])
+
+
+def test_level_engine_refuses_overlong_schedules():
+    """A priority schedule of one item per step has more steps than the
+    level engine launches per sweep: it refuses (before capturing a graph),
+    for one handle and for a batch, and the ready queue runs the same
+    schedule with the level sets' results."""
+    from tycho2_amd.sweeper import SweepBatch
+    print("overlong schedule", flush=True)
+    args = problem(6, 4, 1, seed=97)
+    ref = run_steps(args, FP64, ENGINE_LEVELS, steps=1)
+    sm, conn, omega, wt, st, ss, q = args
+    with SweeperHIP(sm, conn, omega, wt, 1, st, ss, FP64) as s:
+        s.set_priority(1, 0, 1)
+        assert s.info()["n_steps"] == sm.n_cells * omega.shape[0] > 24576
+        s.set_source_isotropic(q)
+        with pytest.raises(T2Error) as e:
+            s.sweep_step()
+        assert e.value.code == -1
+        with SweepBatch([s]) as b:
+            with pytest.raises(T2Error):
+                b.sweep_only()
+            b.set_engine(ENGINE_QUEUE, SPIN)
+            s.set_phi(np.zeros_like(ref[0]))
+            b.sweep_step()
+            b.synchronize()
+        assert np.array_equal(s.phi(), ref[0])
+        s.set_engine(ENGINE_QUEUE, SPIN)
+        s.set_phi(np.zeros_like(ref[0]))
+        s.sweep_step()
+        assert np.array_equal(s.phi(), ref[0])

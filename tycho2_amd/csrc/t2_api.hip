@@ -419,9 +419,31 @@ int after_queue(t2_sweeper *s, int rc)
 
 // psi = L^-1 (src + ang + ang_t): every step of the schedule, captured once
 // per source combination into a hipGraph and replayed.
+// Most step launches one level-engine sweep may issue (one hipGraph node
+// each).  Priority schedules with a small max_cells_per_step can have
+// hundreds of thousands of steps; sweeps of up to this many launches are the
+// ones known to run.
+constexpr int64_t kMaxStepLaunches = 24576;
+
+int check_step_launches(int64_t launches)
+{
+    if (launches > kMaxStepLaunches)
+        return fail(T2_ERR_ARG,
+                    "the level engine issues at most " +
+                        std::to_string(kMaxStepLaunches) +
+                        " step launches per sweep; this schedule needs " +
+                        std::to_string(launches) +
+                        " (use the ready-queue engine or a larger "
+                        "max_cells_per_step)");
+    return T2_OK;
+}
+
 int run_sweep(t2_sweeper *s, const double *ang, const void *ang_t = nullptr)
 {
     T2_TRY(ensure_queue(s));
+    T2_TRY(check_step_launches(s->engine == T2_ENGINE_LEVELS
+                                   ? s->sched.n_steps
+                                   : s->queue_step));
     s->n_sweeps++;
     if (!s->use_graph)
         return enqueue_sweep_any(s, ang, ang_t);
@@ -1910,6 +1932,23 @@ int batch_sweep(t2_batch *b)
         batch_snapshot(b);
     }
     if (b->engine == T2_ENGINE_LEVELS && !b->levels_built) {
+        // one launch per merged step and group count
+        std::vector<std::pair<int64_t, int64_t>> longest;  // (groups, steps)
+        for (auto *m : b->members) {
+            auto it = std::find_if(
+                longest.begin(), longest.end(),
+                [&](const std::pair<int64_t, int64_t> &x) {
+                    return x.first == m->ng;
+                });
+            if (it == longest.end())
+                longest.push_back({m->ng, m->sched.n_steps});
+            else
+                it->second = std::max(it->second, m->sched.n_steps);
+        }
+        int64_t launches = 0;
+        for (auto &x : longest)
+            launches += x.second;
+        T2_TRY(check_step_launches(launches));
         drop_batch_graph(b);
         T2_HIP(hipStreamSynchronize(b->stream));
         T2_TRY(build_levels(b));

@@ -382,54 +382,81 @@ int build_schedule(const HostMesh &m, int64_t na, const double *omega,
 // ------------------------------------------------------------ priorities
 namespace {
 
+// Runs body(a) for every angle on the host's cores.
+template <typename F>
+void for_each_angle(int64_t na, F body)
+{
+    const int nw = n_workers(na);
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+        for (int64_t a; (a = next.fetch_add(1)) < na;)
+            body(a);
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nw; w++)
+        pool.emplace_back(work);
+    work();
+    for (auto &t : pool)
+        t.join();
+}
+
 // The sweep dependency graph of every direction as the level sets resolve
 // it: (a, c) -> (a, u) when face k of c is outgoing (Omega.n > 0, the
 // reference's isOutgoing, src/TychoMesh.hh:107-108), u = adj[c][k], and
-// u's dependency through its face towards c is not lagged.
+// u's dependency through its face towards c is not lagged.  Stored as one
+// byte per (angle, cell): bits 0-3 the downwind faces, bits 4-6 the number
+// of (unlagged) upwind dependencies.
 struct SweepDag {
     const HostMesh &m;
-    int64_t na;
-    const double *omega;
-    std::vector<uint8_t> lag;  // [na][nc]: bit k, face k's dependency lagged
+    std::vector<uint8_t> bits;  // [na][nc]
 
-    SweepDag(const HostMesh &m_, int64_t na_, const double *omega_,
+    SweepDag(const HostMesh &m_, int64_t na, const double *omega,
              const std::vector<LaggedFace> &lagged)
-        : m(m_), na(na_), omega(omega_), lag(size_t(na_) * m_.nc, 0)
+        : m(m_), bits(size_t(na) * m_.nc, 0)
     {
+        const int64_t nc = m.nc;
+        std::vector<uint8_t> lag(size_t(na) * nc, 0);
         for (const LaggedFace &f : lagged)
-            lag[size_t(f.angle) * m.nc + f.cell] |= uint8_t(1u << f.face);
-    }
-
-    bool is_dep(int64_t a, int64_t c, int k) const
-    {
-        return m.adj[4 * c + k] >= 0 &&
-               !(omega_dot_n(omega + 3 * a, &m.normal[(4 * c + k) * 3]) > 0) &&
-               !(lag[size_t(a) * m.nc + c] >> k & 1u);
+            lag[size_t(f.angle) * nc + f.cell] |= uint8_t(1u << f.face);
+        for_each_angle(na, [&](int64_t a) {
+            const double *om = omega + 3 * a;
+            const uint8_t *lg = lag.data() + a * nc;
+            uint8_t *b = bits.data() + a * nc;
+            for (int64_t c = 0; c < nc; c++) {
+                unsigned down = 0, deps = 0;
+                for (int k = 0; k < 4; k++) {
+                    const int64_t u = m.adj[4 * c + k];
+                    if (u < 0)
+                        continue;
+                    if (!(omega_dot_n(om, &m.normal[(4 * c + k) * 3]) > 0)) {
+                        deps += !(lg[c] >> k & 1u);
+                        continue;
+                    }
+                    int back = 0;
+                    while (back < 4 && m.adj[4 * u + back] != c)
+                        back++;
+                    if (back < 4 && !(lg[u] >> back & 1u) &&
+                        !(omega_dot_n(om, &m.normal[(4 * u + back) * 3]) > 0))
+                        down |= 1u << k;
+                }
+                b[c] = uint8_t(down | deps << 4);
+            }
+        });
     }
 
     int down(int64_t a, int64_t c, int64_t out[4]) const
     {
+        const unsigned d = bits[size_t(a) * m.nc + c] & 15u;
         int n = 0;
-        for (int k = 0; k < 4; k++) {
-            const int64_t u = m.adj[4 * c + k];
-            if (u < 0 ||
-                !(omega_dot_n(omega + 3 * a, &m.normal[(4 * c + k) * 3]) > 0))
-                continue;
-            int back = 0;
-            while (back < 4 && m.adj[4 * u + back] != c)
-                back++;
-            if (back < 4 && is_dep(a, u, back))
-                out[n++] = u;
-        }
+        for (int k = 0; k < 4; k++)
+            if (d >> k & 1u)
+                out[n++] = m.adj[4 * c + k];
         return n;
     }
 
     int in_degree(int64_t a, int64_t c) const
     {
-        int n = 0;
-        for (int k = 0; k < 4; k++)
-            n += is_dep(a, c, k);
-        return n;
+        return bits[size_t(a) * m.nc + c] >> 4;
     }
 };
 
@@ -473,9 +500,9 @@ int sweep_priorities(const HostMesh &m, int64_t na, const double *omega,
     // b-level: longest path to a cell with no downwind dependant, by the
     // backward traversal of src/Priorities.cc:120-143 (one rank: no
     // partition sides)
-    std::vector<int32_t> cells;
-    uint64_t max_blevel = 0;
-    for (int64_t a = 0; a < na; a++) {
+    std::vector<uint64_t> top(na, 0);
+    for_each_angle(na, [&](int64_t a) {
+        std::vector<int32_t> cells;
         reverse_level_order(levels + a * nc, nc, cells);
         uint64_t *b = blev.data() + a * nc;
         for (int32_t c : cells) {
@@ -485,9 +512,10 @@ int sweep_priorities(const HostMesh &m, int64_t na, const double *omega,
             for (int i = 0; i < n; i++)
                 v = std::max(v, b[d[i]] + 1);
             b[c] = v;
-            max_blevel = std::max(max_blevel, v);
+            top[a] = std::max(top[a], v);
         }
-    }
+    });
+    const uint64_t max_blevel = *std::max_element(top.begin(), top.end());
     switch (intra) {
     case 0: {  // random (src/Priorities.cc:283-291), a seeded 31-bit draw
         std::mt19937 gen(0);
@@ -504,7 +532,8 @@ int sweep_priorities(const HostMesh &m, int64_t na, const double *omega,
         // rank no face is a partition side, so only the parent shift (0 for
         // BFDS, -1 otherwise, added in unsigned 64-bit arithmetic) acts
         const uint64_t shift = intra == 2 ? 0 : ~uint64_t(0);
-        for (int64_t a = 0; a < na; a++) {
+        for_each_angle(na, [&](int64_t a) {
+            std::vector<int32_t> cells;
             reverse_level_order(levels + a * nc, nc, cells);
             uint64_t *p = prio.data() + a * nc;
             for (int32_t c : cells) {
@@ -515,7 +544,7 @@ int sweep_priorities(const HostMesh &m, int64_t na, const double *omega,
                     v = std::max(v, p[d[i]] + shift);
                 p[c] = v;
             }
-        }
+        });
         break;
     }
     }

@@ -9,7 +9,9 @@
 // SourceIteration::fixedPoint does (src/SourceIteration.cc:127-207), and
 // "DeviceLoop true" in the deck runs the outer iteration on the device;
 // "Precision FP32" and "SweepEngine Queue" select the engine's FP32 sweeps
-// and its persistent ready-queue launch.
+// and its persistent ready-queue launch; "SweepSchedule Priorities" orders
+// the sweep by the reference's traversal priorities (deck keys intraAngleP,
+// interAngleP, maxCellsPerStep) instead of level sets.
 //
 // Usage: sweep_hip.x <mesh.smesh | mesh.pmesh> <input.deck>
 #include <algorithm>
@@ -398,6 +400,13 @@ public:
         const std::string engine = deck.str("SweepEngine", "Levels");
         if (engine != "Levels" && engine != "Queue")
             throw std::runtime_error("SweepEngine must be Levels or Queue");
+        // "SweepSchedule Priorities": the reference's traversal priorities
+        // (src/Priorities.cc:396-435) with maxCellsPerStep ready pairs per
+        // step (src/GraphTraverser.cc:518-536) instead of level sets
+        const std::string sched = deck.str("SweepSchedule", "Levels");
+        if (sched != "Levels" && sched != "Priorities")
+            throw std::runtime_error(
+                "SweepSchedule must be Levels or Priorities");
         // GaussElim as the reference reads it (src/Main.cc:167-176)
         const std::string ge = deck.str("GaussElim", "NoPivot");
         int gauss = T2_GAUSS_NOPIVOT;
@@ -419,6 +428,10 @@ public:
                             engine == "Queue" ? T2_ENGINE_QUEUE
                                               : T2_ENGINE_LEVELS,
                             0, 0));
+        if (sched == "Priorities")
+            check(t2_set_priority(c_engine, int(deck.num("intraAngleP")),
+                                  int(deck.num("interAngleP")),
+                                  int64_t(deck.num("maxCellsPerStep"))));
     }
     ~SweeperHIP() override { t2_sweeper_destroy(c_engine); }
 
