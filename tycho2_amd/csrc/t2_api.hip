@@ -319,11 +319,24 @@ int build_queue(t2_sweeper *s, int64_t first_step)
     const int64_t nq = n_items - first;
     const int64_t per_chunk =
         int64_t(s->block / (s->ng / s->gpl)) * s->chunk_rounds;
+    // a level-set step too small to give every resident block a full chunk
+    // is cut into more, smaller chunks (down to one wavefront round of
+    // items), so that thin levels still spread over the GPU; the steps of a
+    // priority schedule keep full chunks (they are cut by max_cells_per_step
+    // and the next step's items are mostly ready already)
+    int64_t resident = 0;
+    T2_TRY(queue_resident(s, &resident));
+    const int64_t per_wave =
+        s->intra < 0 ? 64 / (s->ng / s->gpl) : per_chunk;
     std::vector<int64_t> off(1, first);
-    for (int64_t st = first_step; st < s->sched.n_steps; st++)
-        for (int64_t i = s->sched.step_off[st]; i < s->sched.step_off[st + 1];
-             i += per_chunk)
-            off.push_back(std::min(i + per_chunk, s->sched.step_off[st + 1]));
+    for (int64_t st = first_step; st < s->sched.n_steps; st++) {
+        const int64_t lo = s->sched.step_off[st], hi = s->sched.step_off[st + 1];
+        const int64_t fair = (hi - lo + resident - 1) / std::max<int64_t>(
+                                                           1, resident);
+        const int64_t c = std::max(per_wave, std::min(per_chunk, fair));
+        for (int64_t i = lo; i < hi; i += c)
+            off.push_back(std::min(i + c, hi));
+    }
     T2_TRY(dalloc_t(s, &s->d_chunk_off, sizeof(int64_t) * off.size()));
     // from here on free_queue releases what was allocated
     s->n_chunks = int64_t(off.size()) - 1;
@@ -1825,19 +1838,34 @@ int build_batch_queue(t2_batch *b)
         }
         probs[i] = batch_problem<T>(m, b->ang_t[i]);
     }
+    // chunks of full size while a merged step gives every resident block
+    // one; thinner steps are cut finer (down to one wavefront round of a
+    // member's items), measured in lanes (items x groups) over all members
+    T2_HIP(t2::queue_batch_grid<T>(&b->grid));
+    const int64_t resident = std::max<int64_t>(1, b->grid);
     std::vector<t2::BatchChunk> chunks;
-    for (int64_t st = 0; st < b->n_steps; st++)
+    for (int64_t st = 0; st < b->n_steps; st++) {
+        int64_t lanes = 0;
+        for (const t2_sweeper *m : b->members)
+            if (st < m->sched.n_steps)
+                lanes += (m->sched.step_off[st + 1] - m->sched.step_off[st]) *
+                         m->ng;
+        const int64_t fair = (lanes + resident - 1) / resident;
         for (int i = 0; i < nm; i++) {
             const t2_sweeper *m = b->members[i];
             if (st >= m->sched.n_steps)
                 continue;
             const int64_t per = int64_t(t2::kMaxBlock / m->ng) *
                                 b->chunk_rounds;
+            const int64_t per_wave = m->intra < 0 ? 64 / m->ng : per;
+            const int64_t c_n = std::max(
+                per_wave, std::min(per, (fair + m->ng - 1) / m->ng));
             const int64_t lo = m->sched.step_off[st];
             const int64_t hi = m->sched.step_off[st + 1];
-            for (int64_t c = lo; c < hi; c += per)
-                chunks.push_back({c, int32_t(std::min(per, hi - c)), i});
+            for (int64_t c = lo; c < hi; c += c_n)
+                chunks.push_back({c, int32_t(std::min(c_n, hi - c)), i});
         }
+    }
     T2_HIP(hipMalloc(&b->d_problems, sizeof(t2::BatchProblem<T>) * nm));
     T2_HIP(hipMemcpy(b->d_problems, probs.data(),
                      sizeof(t2::BatchProblem<T>) * nm, hipMemcpyHostToDevice));
@@ -1853,7 +1881,6 @@ int build_batch_queue(t2_batch *b)
                          sizeof(int32_t)));
         T2_HIP(hipMemset(b->d_abort, 0, sizeof(int32_t)));
     }
-    T2_HIP(t2::queue_batch_grid<T>(&b->grid));
     b->n_chunks = int64_t(chunks.size());
     return T2_OK;
 }

@@ -391,8 +391,8 @@ __device__ __forceinline__ void solve_item(const SweepArgs<T> &p,
             if constexpr (COH)
                 __hip_atomic_store(dst + v * G, b[v], __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
-            else
-                dst[v * G] = b[v];
+            else  // streamed: keeps psi from evicting cached inputs
+                __builtin_nontemporal_store(b[v], dst + v * G);
         }
     }
 }
@@ -451,9 +451,9 @@ __device__ __forceinline__ void queue_chunk(const SweepArgs<T> &p,
     const int wave = int(threadIdx.x / 64), lane = int(threadIdx.x % 64);
     const int gl = lane % LPI;
     const int n_waves = int(blockDim.x / 64);
-    // this wavefront's share of the chunk, a whole number of rounds
-    const int64_t span =
-        ((hi - lo + n_waves - 1) / n_waves + IPR - 1) / IPR * IPR;
+    // this wavefront's share of the chunk: an even split, so that a chunk
+    // smaller than a block round still spreads over every wavefront
+    const int64_t span = (hi - lo + n_waves - 1) / n_waves;
     const int64_t w_lo = lo + wave * span;
     const int64_t w_hi = w_lo + span < hi ? w_lo + span : hi;
     const int64_t first = w_lo + lane / LPI;
