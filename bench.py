@@ -51,8 +51,16 @@ def log(msg):
     print(f"# {msg}", file=sys.stderr, flush=True)
 
 
-def path_leg(p, precision, steps, warmup, ev, tune, solve):
-    """Autotune, time steps and sweeps, and (optionally) solve to 1e-10."""
+def fixed_launch(s, launch):
+    """Apply a "block,groups_per_lane" launch shape (threads per block)."""
+    if launch:
+        block, gpl = (int(x) for x in launch.split(","))
+        s.set_launch(block // (s.ng // gpl), gpl)
+
+
+def path_leg(p, precision, steps, warmup, ev, tune, solve, launch=""):
+    """Autotune (or apply a fixed launch shape), time steps and sweeps, and
+    (optionally) solve to 1e-10."""
     import numpy as np
     from tycho2_amd import autotune
     from tycho2_amd.sweeper import FP32, SweeperHIP
@@ -70,6 +78,8 @@ def path_leg(p, precision, steps, warmup, ev, tune, solve):
                                     engines=(0, 2), log=log)
             out["tuned"] = best
             log(f"autotune: {best}")
+        else:
+            fixed_launch(s, launch)
         info = s.info()
         out["engine"] = info["engine"]
         out["queue_first_step"] = info["queue_first_step"]
@@ -141,12 +151,13 @@ def cpu_leg(p, n_angles, threads):
     return p.nc * n_angles * p.ng / dt, dt
 
 
-def counter_run(p, precision, sweeps):
+def counter_run(p, precision, sweeps, launch=""):
     """A short program for rocprofv3 --pmc: `sweeps` sweeps, then a 4 GiB
     device-to-device copy whose counter readings calibrate the sweep's."""
     from tycho2_amd.sweeper import SweeperHIP
     hip = ctypes.CDLL("libamdhip64.so")
     with SweeperHIP.from_problem(p, precision) as s:
+        fixed_launch(s, launch)
         for _ in range(sweeps):
             s.sweep_only()
         s.synchronize()
@@ -379,11 +390,14 @@ def main() -> int:
                     help="comma-separated waves of the batch workload to run "
                          "(default: all)")
     ap.add_argument("--cpu-angles", type=int, default=0,
-                    help="angles in the CPU-baseline sample (0: 2 x cores)")
+                    help="angles in the CPU-baseline sample (0: 4 x cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the two solves to 1e-10 (steps only)")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--launch", default="",
+                    help="with --no-tune: threads per block and groups per "
+                         "lane of the sweep kernels, e.g. 64,1")
     ap.add_argument("--schedule-study", action="store_true",
                     help="only time level sets against traversal-priority "
                          "schedules on each engine")
@@ -412,7 +426,7 @@ def main() -> int:
         f"{p.solves_per_sweep} solves/sweep (mesh {time.time() - t0:.1f}s)")
     if args.counter_run:
         counter_run(p, FP64 if args.counter_run == "fp64" else FP32,
-                    max(1, args.steps))
+                    max(1, args.steps), args.launch)
         return 0
     ev = HipEvents()
     if args.schedule_study:
@@ -420,10 +434,12 @@ def main() -> int:
         return 0
     solve = not args.no_solve
     tune = not args.no_tune
-    r64 = path_leg(p, FP64, args.steps, args.warmup, ev, tune, solve)
+    r64 = path_leg(p, FP64, args.steps, args.warmup, ev, tune, solve,
+                   args.launch)
     log("fp64: " + str({k: v for k, v in r64.items()
                         if k not in ("phi", "phi_ref")}))
-    r32 = path_leg(p, FP32, args.steps, args.warmup, ev, tune, solve)
+    r32 = path_leg(p, FP32, args.steps, args.warmup, ev, tune, solve,
+                   args.launch)
     log("fp32: " + str({k: v for k, v in r32.items()
                         if k not in ("phi", "phi1")}))
 
@@ -521,7 +537,7 @@ def main() -> int:
            "refined_fp32_fp64": refined}
     if not args.no_cpu:
         cores = min(16, len(os.sched_getaffinity(0)))
-        na_cpu = args.cpu_angles or min(p.na, 2 * cores)
+        na_cpu = args.cpu_angles or min(p.na, 4 * cores)
         rate, dt = cpu_leg(p, na_cpu, cores)
         out["cpu_baseline"] = {
             "value": round(rate, 1), "unit": "solves/s", "cores": cores,

@@ -419,3 +419,31 @@ def test_batch_rejects_changed_member():
     b.close()
     for s in mem:
         s.close()
+
+
+@pytest.mark.parametrize("engine", [0, 1])
+def test_batch_priority_members_bitwise(engine):
+    """Members on priority schedules (the reference's b-level and DFDS
+    rules, two widths) beside a member on level sets: every member's batched
+    steps equal its own, bit for bit."""
+    from tycho2_amd.sweeper import SweepBatch
+    print(f"batch priority members engine={engine}", flush=True)
+    groups = (4, 16, 1)
+    rules = [(1, 0, 256), (3, 1, 1000), (-1, 0, 0)]
+    own = batch_members(FP64, groups)
+    mem = batch_members(FP64, groups)
+    for s, m, r in zip(own, mem, rules):
+        s.set_priority(*r)
+        m.set_priority(*r)
+        for _ in range(2):
+            s.sweep_step()
+    with SweepBatch(mem) as b:
+        b.set_engine(engine, 1 << 20)
+        for _ in range(2):
+            b.sweep_step()
+        b.synchronize()
+    for s, m in zip(own, mem):
+        assert np.array_equal(s.phi(), m.phi())
+        assert np.array_equal(s.psi_angle(1), m.psi_angle(1))
+    for s in own + mem:
+        s.close()
