@@ -258,3 +258,35 @@ def test_level_engine_refuses_overlong_schedules():
         s.set_phi(np.zeros_like(ref[0]))
         s.sweep_step()
         assert np.array_equal(s.phi(), ref[0])
+
+
+def test_autotune_keeps_results_and_applies_best():
+    """The tuner times every knob (engines, angle batches, launch shapes,
+    chunk sizes, priority rules), leaves the fastest on the handle, and the
+    handle's results afterwards equal the untuned ones bit for bit."""
+    from tycho2_amd import autotune
+    print("autotune", flush=True)
+    args = problem(4, 4, 4, seed=98)
+    ref = run_steps(args, FP64, ENGINE_LEVELS)
+    sm, conn, omega, wt, st, ss, q = args
+    with SweeperHIP(sm, conn, omega, wt, 4, st, ss, FP64) as s:
+        best, trials = autotune.tune(s, reps=1, batches=(1, 2),
+                                     blocks=(256, 64),
+                                     priorities=[(1, 0, 256), (3, 1, 0)])
+        assert best == min(trials, key=lambda t: t["ms"])
+        engines = {t["engine"] for t in trials}
+        assert engines == {ENGINE_LEVELS, ENGINE_QUEUE, ENGINE_HYBRID}
+        assert any("priority" in t for t in trials)
+        inf = s.info()
+        assert inf["engine"] == best["engine"]
+        assert inf["angle_batches"] == best["batches"]
+        assert inf["groups_per_lane"] == best["groups_per_lane"]
+        want = best.get("priority", (-1, 0, 0))
+        assert (inf["intra_angle_p"], inf["inter_angle_p"],
+                inf["max_cells_per_step"]) == want
+        s.set_source_isotropic(q)
+        s.set_phi(np.zeros_like(ref[0]))
+        for _ in range(3):
+            s.sweep_step()
+        assert np.array_equal(s.phi(), ref[0])
+        assert np.array_equal(s.psi_angle(omega.shape[0] - 1), ref[1])

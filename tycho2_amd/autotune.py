@@ -9,6 +9,8 @@ stream) over
   * the engine -- per-step launches replayed from a hipGraph, one
     persistent ready-queue launch, or per-step launches with the schedule's
     thin tail on the ready queue (and the queue's chunk size),
+  * optionally, the reference's traversal priorities (intraAngleP,
+    interAngleP, maxCellsPerStep) as the ready queue's order,
 keeps the fastest setting on the handle and returns the measurements.
 Results do not depend on any of these knobs (bitwise, tested).
 """
@@ -31,10 +33,14 @@ def _time_sweep(ev, s, reps):
 
 def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
          groups_per_lane=GROUPS_PER_LANE,
-         engines=(ENGINE_LEVELS, ENGINE_QUEUE, ENGINE_HYBRID), log=None):
+         engines=(ENGINE_LEVELS, ENGINE_QUEUE, ENGINE_HYBRID), log=None,
+         priorities=()):
     """Tune handle `s` in place; returns (best, trials) where best is a dict
-    {engine, batches, block, groups_per_lane, ms} and trials lists every
-    timing; log (a callable) receives one line per trial."""
+    {engine, batches, block, groups_per_lane, ms[, chunk_rounds,
+    priority]} and trials lists every timing; log (a callable) receives one
+    line per trial.  priorities: (intraAngleP, interAngleP, maxCellsPerStep)
+    rules tried on the ready queue after the rest (each rebuilds the
+    schedule on the host)."""
     ev = HipEvents()
     ng = s.ng
     trials = []
@@ -91,6 +97,27 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
                     log(f"tune {t}")
                 if ms < best["ms"]:
                     best = t
+    # the reference's traversal priorities as the queue's order, at the best
+    # chunk size found for the queue (level sets otherwise)
+    if priorities:
+        q_trials = [t for t in trials if t["engine"] == ENGINE_QUEUE]
+        rounds = (min(q_trials, key=lambda t: t["ms"])["chunk_rounds"]
+                  if q_trials else 0)
+        s.set_engine(ENGINE_QUEUE, chunk_rounds=rounds)
+        s.set_schedule(1)
+        for rule in priorities:
+            s.set_priority(*rule)
+            ms = _time_sweep(ev, s, reps)
+            t = dict(engine=ENGINE_QUEUE, batches=1,
+                     block=level_best["block"],
+                     groups_per_lane=level_best["groups_per_lane"],
+                     chunk_rounds=rounds, priority=tuple(rule), ms=ms)
+            trials.append(t)
+            if log:
+                log(f"tune {t}")
+            if ms < best["ms"]:
+                best = t
+        s.set_priority(*best.get("priority", (-1, 0, 0)))
     s.set_engine(best["engine"], chunk_rounds=best.get("chunk_rounds", 0))
     s.set_schedule(best["batches"])
     return best, trials
