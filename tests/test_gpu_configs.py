@@ -75,3 +75,27 @@ def test_scattering_config_refined_meets_1e10():
           f"FP32 stage {e1:.2e}, refined {e2:.2e}", flush=True)
     assert e2 <= 1e-10
     assert e1 > 10 * e2
+
+
+def test_plain_config_priority_schedules_bitwise():
+    """At full size the ready queue in b-level order and the level engine in
+    DFDS order (the reference's default deck rules) give the level sets'
+    source-iteration step bit for bit."""
+    from tycho2_amd.sweeper import ENGINE_LEVELS, ENGINE_QUEUE
+    t0 = time.time()
+    p = problems.plain()
+    stage(f"plain: {p.nc} cells x {p.na} angles x {p.ng} groups", t0)
+    with SweeperHIP.from_problem(p, FP64) as s:
+        s.sweep_step()
+        ref = s.phi()
+        ref_psi = s.psi_angle(p.na - 1)
+        for engine, rule in ((ENGINE_QUEUE, (1, 0, 32768)),
+                             (ENGINE_LEVELS, (3, 1, 262144))):
+            s.set_priority(*rule)
+            s.set_engine(engine)
+            stage(f"engine {engine} priority {rule}: "
+                  f"{s.info()['n_steps']} steps", t0)
+            s.set_phi(np.zeros_like(ref))
+            s.sweep_step()
+            assert np.array_equal(s.phi(), ref)
+            assert np.array_equal(s.psi_angle(p.na - 1), ref_psi)
