@@ -359,6 +359,43 @@ def schedule_leg(p, args, ev):
     print(json.dumps(res), flush=True)
 
 
+def host_leg(p, args):
+    """The drop-in path: t2_sweep_host on host arrays in the reference
+    layout (source uploaded, swept, psi downloaded; ordinary pageable
+    memory, as the reference's PsiData is), wall-clock timed."""
+    import numpy as np
+    from tycho2_amd.sweeper import FP64, SweeperHIP
+    shape = (p.nc, p.na, 4, p.ng)
+    src = np.full(shape, 1.0)
+    psi = np.zeros(shape)
+    with SweeperHIP.from_problem(p, FP64) as s:
+        s.sweep(psi, src)                        # warm-up: staging buffers
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            s.sweep(psi, src)
+        sec = (time.perf_counter() - t0) / args.steps
+        s.sweep_only()
+        s.synchronize()
+        from tycho2_amd.timing import HipEvents, time_region
+        ev = HipEvents()
+        dev_ms = time_region(ev, s, s.sweep_only, args.steps) / args.steps
+    moved = 2 * psi.nbytes
+    res = {"metric": "DG cell-angle-group solves/s of the drop-in host call "
+                     "(t2_sweep_host, PCIe-inclusive), FP64",
+           "value": round(p.solves_per_sweep / sec), "unit": "solves/s",
+           "n_gpus": 1, "steps": args.steps, "higher_is_better": True,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{args.config}: {p.nc} tets, {p.na} "
+                                  f"angles, {p.ng} groups"},
+           "seconds_per_call": round(sec, 4),
+           "host_bytes_moved_per_call": moved,
+           "host_link_gb_per_s": round(moved / sec / 1e9, 2),
+           "device_sweep_ms": round(dev_ms, 3),
+           "device_only_solves_per_s": round(p.solves_per_sweep /
+                                             (dev_ms * 1e-3))}
+    print(json.dumps(res), flush=True)
+
+
 def traffic_from_profiles(workload, precision):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
@@ -398,6 +435,8 @@ def main() -> int:
     ap.add_argument("--launch", default="",
                     help="with --no-tune: threads per block and groups per "
                          "lane of the sweep kernels, e.g. 64,1")
+    ap.add_argument("--host-path", action="store_true",
+                    help="only time the drop-in host call (PCIe-inclusive)")
     ap.add_argument("--schedule-study", action="store_true",
                     help="only time level sets against traversal-priority "
                          "schedules on each engine")
@@ -431,6 +470,9 @@ def main() -> int:
     ev = HipEvents()
     if args.schedule_study:
         schedule_leg(p, args, ev)
+        return 0
+    if args.host_path:
+        host_leg(p, args)
         return 0
     solve = not args.no_solve
     tune = not args.no_tune
