@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "t2_common.hpp"
@@ -49,6 +50,8 @@ struct t2_sweeper {
     int intra = -1, inter = 0;       // priority schedule (intra < 0: levels)
     int64_t step_width = 0;
     bool use_graph = true;
+    void *h_pin[2] = {nullptr, nullptr};   // pinned host staging (host link)
+    hipEvent_t pin_ev[2] = {nullptr, nullptr};
     t2::HostMesh hm;                 // kept for schedule rebuilds
     std::vector<double> omega_h, sigma_t_h;
     t2::Schedule sched;
@@ -518,6 +521,98 @@ int zero_psi(t2_sweeper *s)
     const int64_t n = int64_t(s->n_psi() * s->tsize() / 8);
     T2_HIP(t2::launch_fill(static_cast<double *>(s->d_psi), 0.0, n,
                            s->stream));
+    return T2_OK;
+}
+
+// ---------------------------------------------------------- host link
+// psi-sized transfers between the caller's ordinary (pageable) arrays and
+// the device go through two pinned chunks: host threads copy one chunk
+// while the DMA engine moves the other, instead of the runtime's serial
+// staging of pageable memory.
+constexpr size_t kPinChunk = size_t(64) << 20;
+
+int ensure_pinned(t2_sweeper *s)
+{
+    for (int b = 0; b < 2; b++) {
+        if (!s->h_pin[b])
+            T2_HIP(hipHostMalloc(&s->h_pin[b], kPinChunk));
+        if (!s->pin_ev[b])
+            T2_HIP(hipEventCreateWithFlags(&s->pin_ev[b],
+                                           hipEventDisableTiming));
+    }
+    return T2_OK;
+}
+
+void parallel_copy(void *dst, const void *src, size_t bytes)
+{
+    const size_t min_part = size_t(4) << 20;
+    unsigned hw = std::thread::hardware_concurrency();
+    size_t parts = std::min<size_t>(hw ? std::min(hw, 8u) : 4u,
+                                    std::max<size_t>(1, bytes / min_part));
+    if (parts <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t step = (bytes + parts - 1) / parts;
+    std::vector<std::thread> pool;
+    for (size_t i = 1; i < parts; i++) {
+        const size_t lo = i * step;
+        if (lo >= bytes)
+            break;
+        const size_t len = std::min(step, bytes - lo);
+        pool.emplace_back([=]() {
+            std::memcpy(static_cast<char *>(dst) + lo,
+                        static_cast<const char *>(src) + lo, len);
+        });
+    }
+    std::memcpy(dst, src, std::min(step, bytes));
+    for (auto &t : pool)
+        t.join();
+}
+
+// Host -> device, ordered on the handle's stream (later work sees it).
+int upload_host(t2_sweeper *s, void *dev, const void *host, size_t bytes)
+{
+    T2_TRY(ensure_pinned(s));
+    for (size_t off = 0, k = 0; off < bytes; off += kPinChunk, k++) {
+        const size_t len = std::min(kPinChunk, bytes - off);
+        const int b = int(k & 1);
+        T2_HIP(hipEventSynchronize(s->pin_ev[b]));   // its last DMA is done
+        parallel_copy(s->h_pin[b], static_cast<const char *>(host) + off, len);
+        T2_HIP(hipMemcpyAsync(static_cast<char *>(dev) + off, s->h_pin[b],
+                              len, hipMemcpyHostToDevice, s->stream));
+        T2_HIP(hipEventRecord(s->pin_ev[b], s->stream));
+    }
+    return T2_OK;
+}
+
+// Device -> host after all earlier work on the handle's stream; returns
+// when the host array is complete.
+int download_host(t2_sweeper *s, void *host, const void *dev, size_t bytes)
+{
+    T2_TRY(ensure_pinned(s));
+    size_t prev_off = 0, prev_len = 0;
+    int prev_b = -1;
+    for (size_t off = 0, k = 0; off < bytes; off += kPinChunk, k++) {
+        const size_t len = std::min(kPinChunk, bytes - off);
+        const int b = int(k & 1);
+        T2_HIP(hipMemcpyAsync(s->h_pin[b], static_cast<const char *>(dev) + off,
+                              len, hipMemcpyDeviceToHost, s->stream));
+        T2_HIP(hipEventRecord(s->pin_ev[b], s->stream));
+        if (prev_b >= 0) {
+            T2_HIP(hipEventSynchronize(s->pin_ev[prev_b]));
+            parallel_copy(static_cast<char *>(host) + prev_off,
+                          s->h_pin[prev_b], prev_len);
+        }
+        prev_off = off;
+        prev_len = len;
+        prev_b = b;
+    }
+    if (prev_b >= 0) {
+        T2_HIP(hipEventSynchronize(s->pin_ev[prev_b]));
+        parallel_copy(static_cast<char *>(host) + prev_off, s->h_pin[prev_b],
+                      prev_len);
+    }
     return T2_OK;
 }
 
@@ -1044,6 +1139,12 @@ int t2_sweeper_destroy(t2_sweeper *s)
     drop_graphs(s);
     for (void *p : s->allocs)
         (void)hipFree(p);
+    for (int b = 0; b < 2; b++) {
+        if (s->h_pin[b])
+            (void)hipHostFree(s->h_pin[b]);
+        if (s->pin_ev[b])
+            (void)hipEventDestroy(s->pin_ev[b]);
+    }
     if (s->stream)
         (void)hipStreamDestroy(s->stream);
     delete s;
@@ -1175,8 +1276,7 @@ static int sweep_host_body(t2_sweeper *s, double *psi, const double *source,
         T2_TRY(dalloc_t(s, &s->d_stage, sizeof(double) * n));
     if (!s->d_hostang)
         T2_TRY(dalloc_t(s, &s->d_hostang, sizeof(double) * n));
-    T2_HIP(hipMemcpyAsync(s->d_stage, source, sizeof(double) * n,
-                          hipMemcpyHostToDevice, s->stream));
+    T2_TRY(upload_host(s, s->d_stage, source, sizeof(double) * n));
     T2_HIP(t2::launch_ca_to_ac(s->d_stage, s->d_hostang, s->nc, s->na, vg,
                                s->stream));
     if (!s->sched.lagged.empty()) {
@@ -1184,8 +1284,7 @@ static int sweep_host_body(t2_sweeper *s, double *psi, const double *source,
         if (s->precision != T2_FP64)
             return fail(T2_ERR_STATE,
                         "host sweep with lagged faces needs FP64 storage");
-        T2_HIP(hipMemcpyAsync(s->d_stage, psi, sizeof(double) * n,
-                              hipMemcpyHostToDevice, s->stream));
+        T2_TRY(upload_host(s, s->d_stage, psi, sizeof(double) * n));
         T2_HIP(t2::launch_ca_to_ac(s->d_stage,
                                    static_cast<double *>(s->d_psi), s->nc,
                                    s->na, vg, s->stream));
@@ -1200,7 +1299,7 @@ static int sweep_host_body(t2_sweeper *s, double *psi, const double *source,
         T2_HIP(t2::launch_ac_to_ca<float>(static_cast<const float *>(s->d_psi),
                                           s->d_stage, s->nc, s->na, vg,
                                           s->stream));
-    return fetch(s, psi, s->d_stage, n);
+    return download_host(s, psi, s->d_stage, sizeof(double) * n);
 }
 
 int t2_set_source_isotropic(t2_sweeper *s, const double *q)
@@ -1233,8 +1332,7 @@ int t2_set_source_angular(t2_sweeper *s, const double *source)
     T2_TRY(dalloc_t(s, &s->d_qang, sizeof(double) * n));
     if (!s->d_stage)
         T2_TRY(dalloc_t(s, &s->d_stage, sizeof(double) * n));
-    T2_HIP(hipMemcpyAsync(s->d_stage, source, sizeof(double) * n,
-                          hipMemcpyHostToDevice, s->stream));
+    T2_TRY(upload_host(s, s->d_stage, source, sizeof(double) * n));
     T2_HIP(t2::launch_ca_to_ac(s->d_stage, s->d_qang, s->nc, s->na,
                                4 * s->ng, s->stream));
     T2_HIP(hipStreamSynchronize(s->stream));
@@ -1429,7 +1527,7 @@ int t2_get_psi(const t2_sweeper *s_const, double *psi)
         T2_HIP(t2::launch_ac_to_ca<float>(static_cast<const float *>(s->d_psi),
                                           s->d_stage, s->nc, s->na, 4 * s->ng,
                                           s->stream));
-    return fetch(s, psi, s->d_stage, n);
+    return download_host(s, psi, s->d_stage, sizeof(double) * n);
 }
 
 int t2_get_psi_angle(const t2_sweeper *s, int64_t angle, double *psi_a)
