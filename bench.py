@@ -187,7 +187,7 @@ def batch_leg(args, ev):
     import numpy as np
     from tycho2_amd import problems
     from tycho2_amd.sweeper import (ENGINE_LEVELS, ENGINE_QUEUE, FP64,
-                                    SweepBatch, SweeperHIP, T2Error)
+                                    SweepBatch, SweeperHIP)
     from tycho2_amd.timing import time_region
     t0 = time.time()
     ps = problems.batch(count=args.batch_count)
@@ -220,36 +220,31 @@ def batch_leg(args, ev):
             f"{time.time() - t1:.0f}s")
         solves = sum(p.solves_per_sweep for p in wave)
         rec = dict(problems=len(wave), solves_per_sweep=solves)
-        def timed(b, name, engine, rounds):
-            b.set_engine(engine, 0, rounds)
-            try:
-                b.sweep_only()
-            except T2Error as e:
-                # the level form refuses schedules of too many steps
-                log(f"wave {w}: {name} not run: {e}")
-                return float("inf")
-            b.synchronize()
-            ms = time_region(ev, b, b.sweep_only, args.steps) / args.steps
-            log(f"wave {w}: {name} {ms:.2f} ms/sweep, "
-                f"{solves / ms / 1e6:.1f} G solves/s")
-            return ms
+        # engines and queue chunk sizes timed by the project's autotuner
+        from tycho2_amd import autotune
 
-        timings = {}
+        def tuned(b, prefix, engines):
+            _, trials = autotune.tune_batch(b, reps=args.steps,
+                                            engines=engines, log=log)
+            out = {}
+            for t in trials:
+                name = ("levels" if t["engine"] == ENGINE_LEVELS
+                        else f"{prefix}_r{t['chunk_rounds']}")
+                out[name] = t["ms"]
+                log(f"wave {w}: {name} {t['ms']:.2f} ms/sweep, "
+                    f"{solves / t['ms'] / 1e6:.1f} G solves/s")
+            return out
+
+        timings = {"levels": float("inf")}
         with SweepBatch(hs) as b:
-            for name, engine, rounds in (("levels", ENGINE_LEVELS, 0),
-                                         ("queue_r2", ENGINE_QUEUE, 2),
-                                         ("queue_r4", ENGINE_QUEUE, 4),
-                                         ("queue_r8", ENGINE_QUEUE, 8)):
-                timings[name] = timed(b, name, engine, rounds)
+            timings.update(tuned(b, "queue", (ENGINE_LEVELS, ENGINE_QUEUE)))
         if rule:
             # the ready queue in the order of the reference's traversal
             # priorities (members re-scheduled, so a new batch)
             for h in hs:
                 h.set_priority(*rule)
             with SweepBatch(hs) as b:
-                for name, rounds in (("queue_prio_r2", 2),
-                                     ("queue_prio_r4", 4)):
-                    timings[name] = timed(b, name, ENGINE_QUEUE, rounds)
+                timings.update(tuned(b, "queue_prio", (ENGINE_QUEUE,)))
         best = min(timings, key=timings.get)
         if rule and not best.startswith("queue_prio"):
             for h in hs:
@@ -268,7 +263,8 @@ def batch_leg(args, ev):
         tot["solves"] += solves
         with SweepBatch(hs) as b:
             b.set_engine(ENGINE_LEVELS if best == "levels" else ENGINE_QUEUE,
-                         0, 0 if best == "levels" else int(best[-1]))
+                         0, 0 if best == "levels"
+                         else int(best.rsplit("_r", 1)[1]))
             t2 = time.time()
             its, sec = b.solve_krylov(300, 1e-10, 20)
             rec["solve_s"] = round(time.time() - t2, 3)

@@ -447,3 +447,30 @@ def test_batch_priority_members_bitwise(engine):
         assert np.array_equal(s.psi_angle(1), m.psi_angle(1))
     for s in own + mem:
         s.close()
+
+
+def test_batch_autotune_keeps_results():
+    """The batch tuner times both engines and the queue's chunk sizes,
+    leaves the fastest on the batch, and the batch's steps afterwards equal
+    the members' own bit for bit."""
+    from tycho2_amd import autotune
+    from tycho2_amd.sweeper import SweepBatch
+    print("batch autotune", flush=True)
+    groups = (4, 16, 1)
+    own = batch_members(FP64, groups)
+    mem = batch_members(FP64, groups)
+    for s in own:
+        s.sweep_step()
+    with SweepBatch(mem) as b:
+        best, trials = autotune.tune_batch(b, reps=1)
+        assert best == min(trials, key=lambda t: t["ms"])
+        assert {t["engine"] for t in trials} == {0, 1}
+        assert len(trials) == 1 + len(autotune.CHUNK_ROUNDS)
+        for m in mem:
+            m.set_phi(np.zeros_like(m.phi()))
+        b.sweep_step()
+        b.synchronize()
+    for s, m in zip(own, mem):
+        assert np.array_equal(s.phi(), m.phi())
+    for s in own + mem:
+        s.close()

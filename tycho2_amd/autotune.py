@@ -121,3 +121,34 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
     s.set_engine(best["engine"], chunk_rounds=best.get("chunk_rounds", 0))
     s.set_schedule(best["batches"])
     return best, trials
+
+
+def tune_batch(b, reps: int = 3, rounds=CHUNK_ROUNDS,
+               engines=(ENGINE_LEVELS, ENGINE_QUEUE), log=None):
+    """Tune batch `b` in place over its engine (merged per-step launches or
+    one persistent ready-queue launch) and the queue's chunk size; returns
+    (best, trials) like tune().  A level form that refuses the members'
+    schedules (too many steps) is skipped."""
+    from .sweeper import T2Error
+    ev = HipEvents()
+    trials = []
+    for engine in engines:
+        for r in (rounds if engine == ENGINE_QUEUE else (0,)):
+            b.set_engine(engine, 0, r)
+            try:
+                b.sweep_only()
+            except T2Error as e:
+                if log:
+                    log(f"tune batch engine={engine}: {e}")
+                continue
+            b.synchronize()
+            ms = time_region(ev, b, b.sweep_only, reps) / reps
+            t = dict(engine=engine, chunk_rounds=r, ms=ms)
+            trials.append(t)
+            if log:
+                log(f"tune batch {t}")
+    if not trials:
+        raise RuntimeError("no batch engine could sweep these members")
+    best = min(trials, key=lambda t: t["ms"])
+    b.set_engine(best["engine"], 0, best["chunk_rounds"])
+    return best, trials
