@@ -51,14 +51,19 @@ def log(msg):
     print(f"# {msg}", file=sys.stderr, flush=True)
 
 
-def fixed_launch(s, launch):
-    """Apply a "block,groups_per_lane" launch shape (threads per block)."""
+def fixed_launch(s, launch, batches=1, concurrent=False):
+    """Apply a "block,groups_per_lane" launch shape (threads per block) and
+    an angle-batch schedule (one after the other or concurrent)."""
     if launch:
         block, gpl = (int(x) for x in launch.split(","))
         s.set_launch(block // (s.ng // gpl), gpl)
+    if batches > 1:
+        s.set_schedule(batches)
+    s.set_concurrent_batches(concurrent)
 
 
-def path_leg(p, precision, steps, warmup, ev, tune, solve, launch=""):
+def path_leg(p, precision, steps, warmup, ev, tune, solve, launch="",
+             batches=1, concurrent=False):
     """Autotune (or apply a fixed launch shape), time steps and sweeps, and
     (optionally) solve to 1e-10."""
     import numpy as np
@@ -79,10 +84,12 @@ def path_leg(p, precision, steps, warmup, ev, tune, solve, launch=""):
             out["tuned"] = best
             log(f"autotune: {best}")
         else:
-            fixed_launch(s, launch)
+            fixed_launch(s, launch, batches, concurrent)
         info = s.info()
         out["engine"] = info["engine"]
         out["queue_first_step"] = info["queue_first_step"]
+        out["streams"] = (info["angle_batches"]
+                          if info["concurrent_batches"] else 1)
         for _ in range(warmup):
             s.sweep_step()
         s.synchronize()
@@ -151,13 +158,14 @@ def cpu_leg(p, n_angles, threads):
     return p.nc * n_angles * p.ng / dt, dt
 
 
-def counter_run(p, precision, sweeps, launch=""):
+def counter_run(p, precision, sweeps, launch="", batches=1,
+                concurrent=False):
     """A short program for rocprofv3 --pmc: `sweeps` sweeps, then a 4 GiB
     device-to-device copy whose counter readings calibrate the sweep's."""
     from tycho2_amd.sweeper import SweeperHIP
     hip = ctypes.CDLL("libamdhip64.so")
     with SweeperHIP.from_problem(p, precision) as s:
-        fixed_launch(s, launch)
+        fixed_launch(s, launch, batches, concurrent)
         for _ in range(sweeps):
             s.sweep_only()
         s.synchronize()
@@ -487,12 +495,21 @@ def batch_refined_leg(args, ev):
     print(json.dumps(res), flush=True)
 
 
-def traffic_from_profiles(workload, precision):
+def traffic_from_profiles(workload, precision, launches_per_sweep=None):
+    """Measured HBM bytes of one sweep-kernel launch: the committed
+    counter figures per sweep, over this run's launches per sweep (a
+    different angle-batch schedule splits the same traffic over more
+    launches)."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get(precision, {}).get("hbm_bytes_per_launch")
+        rec = json.load(f).get(precision)
+    if not rec:
+        return None
+    if launches_per_sweep is None:
+        return rec.get("hbm_bytes_per_launch")
+    return round(rec["hbm_bytes_per_sweep"] / launches_per_sweep)
 
 
 def maxrel(a, b):
@@ -526,6 +543,11 @@ def main() -> int:
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the two solves to 1e-10 (steps only)")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--batches", type=int, default=1,
+                    help="with --no-tune: angle batches of the schedule")
+    ap.add_argument("--concurrent", action="store_true",
+                    help="with --no-tune: sweep the angle batches on "
+                         "parallel streams")
     ap.add_argument("--launch", default="",
                     help="with --no-tune: threads per block and groups per "
                          "lane of the sweep kernels, e.g. 64,1")
@@ -562,7 +584,8 @@ def main() -> int:
         f"{p.solves_per_sweep} solves/sweep (mesh {time.time() - t0:.1f}s)")
     if args.counter_run:
         counter_run(p, FP64 if args.counter_run == "fp64" else FP32,
-                    max(1, args.steps), args.launch)
+                    max(1, args.steps), args.launch, args.batches,
+                    args.concurrent)
         return 0
     ev = HipEvents()
     if args.schedule_study:
@@ -574,11 +597,11 @@ def main() -> int:
     solve = not args.no_solve
     tune = not args.no_tune
     r64 = path_leg(p, FP64, args.steps, args.warmup, ev, tune, solve,
-                   args.launch)
+                   args.launch, args.batches, args.concurrent)
     log("fp64: " + str({k: v for k, v in r64.items()
                         if k not in ("phi", "phi_ref")}))
     r32 = path_leg(p, FP32, args.steps, args.warmup, ev, tune, solve,
-                   args.launch)
+                   args.launch, args.batches, args.concurrent)
     log("fp32: " + str({k: v for k, v in r32.items()
                         if k not in ("phi", "phi1")}))
 
@@ -599,17 +622,24 @@ def main() -> int:
         else:
             kernel = f"sweep_step_kernel<{ty},{p.ng},{gpl}>"
             launches = r["n_steps"]
+        # concurrent angle batches: `streams` chains of launches
+        streams = r.get("streams", 1) if r.get("engine", 0) == 0 else 1
         ach = solves * bytes_per_solve / (r["ms_per_sweep"] * 1e-3) / 1e9
         return {"bound": "hbm", "kernel": kernel, "achieved": round(ach, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profiles(args.config,
-                                                 f"fp{8 * width}"),
+                                                 f"fp{8 * width}",
+                                                 launches),
                 "alg_bytes_per_solve": round(bytes_per_solve, 3),
                 "alg_bytes_per_launch": round(solves * bytes_per_solve /
                                               launches),
+                # the sweep's time per launch: with `streams` concurrent
+                # chains the launches overlap, so this is the interval, not
+                # one kernel's duration
                 "avg_launch_ms": round(r["ms_per_sweep"] / launches, 5),
-                "launches_per_sweep": launches}
+                "launches_per_sweep": launches,
+                "streams": streams}
 
     plain = {"solves_per_s_step": round(solves / (r64["ms_per_step"] * 1e-3)),
              "solves_per_s_sweep": round(solves /

@@ -82,6 +82,7 @@ typedef struct {
     int32_t intra_angle_p;    /* priority schedule rules (t2_set_priority; */
     int32_t inter_angle_p;    /*   -1 and 0: level sets)                   */
     int64_t max_cells_per_step;
+    int32_t concurrent_batches; /* angle batches on parallel streams     */
 } t2_info;
 
 /* Message of the last failure on this thread ("" if none). */
@@ -168,6 +169,15 @@ int t2_set_launch(t2_sweeper *s, int items_per_block, int groups_per_lane);
  * working set so that upwind reads stay in cache (autotuner knob).  Results
  * do not depend on it. */
 int t2_set_schedule(t2_sweeper *s, int angle_batches);
+
+/* Level engine: sweep the angle batches of t2_set_schedule concurrently,
+ * batch b > 0 on a stream of its own forked from and joined back to the
+ * handle's stream (parallel branches of the captured hipGraph), so that one
+ * batch's launch gaps are filled with another's work; 0 (default) sweeps
+ * them one after the other.  Batches are independent within a sweep, so
+ * results do not depend on it.  No reference counterpart (the reference's
+ * angle groups are OpenMP threads, src/GraphTraverser.cc:151-176). */
+int t2_set_concurrent_batches(t2_sweeper *s, int concurrent);
 
 /* Order of the items within a step (any order is valid; results do not
  * depend on it): T2_ORDER_CELL by angle then cell; T2_ORDER_FACES (default)

@@ -57,8 +57,12 @@ def test_traffic_figures_from_profiles():
     for prec in ("fp64", "fp32"):
         t = bench.traffic_from_profiles("precision", prec)
         assert t == prof[prec]["hbm_bytes_per_launch"]
-        # actual traffic at least the algorithmic bytes of a launch
-        assert t >= prof[prec]["algorithmic_bytes_per_launch"]
+        # a schedule of 867 launches splits the same per-sweep traffic
+        t867 = bench.traffic_from_profiles("precision", prec, 867)
+        assert t867 == round(prof[prec]["hbm_bytes_per_sweep"] / 867)
+        # actual traffic at least the algorithmic bytes
+        assert (prof[prec]["hbm_bytes_per_sweep"] >=
+                prof[prec]["algorithmic_bytes_per_sweep"])
     assert bench.traffic_from_profiles("no-such-workload", "fp64") is None
 
 

@@ -281,6 +281,7 @@ def test_autotune_keeps_results_and_applies_best():
         assert inf["engine"] == best["engine"]
         assert inf["angle_batches"] == best["batches"]
         assert inf["groups_per_lane"] == best["groups_per_lane"]
+        assert inf["concurrent_batches"] == int(best["concurrent"])
         want = best.get("priority", (-1, 0, 0))
         assert (inf["intra_angle_p"], inf["inter_angle_p"],
                 inf["max_cells_per_step"]) == want
@@ -290,3 +291,62 @@ def test_autotune_keeps_results_and_applies_best():
             s.sweep_step()
         assert np.array_equal(s.phi(), ref[0])
         assert np.array_equal(s.psi_angle(omega.shape[0] - 1), ref[1])
+
+
+@pytest.mark.parametrize("precision,batches", [(FP64, 2), (FP32, 2),
+                                               (FP64, 4)])
+def test_concurrent_angle_batches_bitwise(precision, batches):
+    """Angle batches swept side by side on parallel streams (forked and
+    joined inside the captured graph) give the one-batch sweep's results,
+    also with lagged faces and through a GMRES solve."""
+    print(f"concurrent batches precision={precision} batches={batches}",
+          flush=True)
+    args = problem(5, 4, 16, seed=99)
+    ref = run_steps(args, precision, ENGINE_LEVELS)
+    sm, conn, omega, wt, st, ss, q = args
+    with SweeperHIP(sm, conn, omega, wt, 16, st, ss, precision) as s:
+        s.set_schedule(batches)
+        s.set_concurrent_batches(True)
+        assert s.info()["concurrent_batches"] == 1
+        s.set_source_isotropic(q)
+        for _ in range(3):
+            s.sweep_step()
+        assert np.array_equal(s.phi(), ref[0])
+        assert np.array_equal(s.psi_angle(omega.shape[0] - 1), ref[1])
+    # cyclic mesh: lagged faces read the pre-sweep psi in every branch
+    sm = meshmod.kuhn_cube(3, 0.2, seed=3)
+    conn = meshmod.tycho_connectivity(sm)
+    omega, wt = orc.quadrature(2)
+    rng = np.random.default_rng(10)
+    st = rng.uniform(1.0, 2.0, (sm.n_cells, 1))
+    src = rng.uniform(0.1, 1.0, (sm.n_cells, omega.shape[0], 4, 1))
+    prev = rng.uniform(0.0, 1.0, src.shape)
+    psi = prev.copy()
+    with SweeperHIP(sm, conn, omega, wt, 1, st, 0.5 * st, FP64) as s:
+        s.set_schedule(2)
+        s.set_concurrent_batches(True)
+        assert s.info()["n_lagged"] > 0
+        s.sweep(psi, src)
+    ref_psi, _ = orc.sweep(orc.build_mesh(sm), omega, 1, st, src,
+                           psi=prev.copy())
+    assert np.abs(psi - ref_psi).max() <= 1e-12 * np.abs(ref_psi).max()
+
+
+def test_concurrent_batches_krylov_and_no_graph_bitwise(monkeypatch):
+    """The same fork/join also runs uncaptured (direct launches), and a
+    GMRES solve on concurrent batches equals the one-batch solve."""
+    print("concurrent batches krylov", flush=True)
+    sm, conn, omega, wt, st, ss, q = problem(4, 4, 4, seed=61)
+    out = []
+    for conc, no_graph in ((False, "0"), (True, "0"), (True, "1")):
+        monkeypatch.setenv("T2_NO_GRAPH", no_graph)
+        with SweeperHIP(sm, conn, omega, wt, 4, st, ss, FP64) as s:
+            if conc:
+                s.set_schedule(2)
+                s.set_concurrent_batches(True)
+            s.set_source_isotropic(q)
+            hist = s.solve_krylov(200, 1e-10, 20)
+            out.append((s.phi(), len(hist)))
+    for phi, n in out[1:]:
+        assert n == out[0][1]
+        assert np.array_equal(phi, out[0][0])

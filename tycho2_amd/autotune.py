@@ -3,7 +3,8 @@
 For one sweeper handle it times full sweeps (HIP events on the engine's
 stream) over
   * angle batches  -- how many octant-ordered groups of angles are swept one
-    after the other (smaller per-step working set, more steps),
+    after the other (smaller per-step working set, more steps), or side by
+    side on parallel streams,
   * cells per block and groups per lane -- the launch shape of the sweep
     kernels,
   * the engine -- per-step launches replayed from a hipGraph, one
@@ -46,21 +47,25 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
     trials = []
     best = None
     s.set_engine(ENGINE_LEVELS)
-    # schedule first (it rebuilds the records), with the default launch
+    # schedule first (it rebuilds the records), with the default launch;
+    # several angle batches one after the other or side by side
     for nb in batches:
         if nb > s.na:
             continue
         s.set_schedule(nb)
         s.set_launch(256 // ng if ng <= 256 else 1, 1)
-        ms = _time_sweep(ev, s, reps)
-        t = dict(engine=ENGINE_LEVELS, batches=nb, block=256,
-                 groups_per_lane=1, ms=ms)
-        trials.append(t)
-        if log:
-            log(f"tune {t}")
-        if best is None or ms < best["ms"]:
-            best = t
+        for conc in ((False, True) if nb > 1 else (False,)):
+            s.set_concurrent_batches(conc)
+            ms = _time_sweep(ev, s, reps)
+            t = dict(engine=ENGINE_LEVELS, batches=nb, concurrent=conc,
+                     block=256, groups_per_lane=1, ms=ms)
+            trials.append(t)
+            if log:
+                log(f"tune {t}")
+            if best is None or ms < best["ms"]:
+                best = t
     s.set_schedule(best["batches"])
+    s.set_concurrent_batches(best["concurrent"])
     for gpl in groups_per_lane:
         if gpl > ng:
             continue
@@ -71,7 +76,8 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
             s.set_launch(block // lanes, gpl)
             ms = _time_sweep(ev, s, reps)
             t = dict(engine=ENGINE_LEVELS, batches=best["batches"],
-                     block=block, groups_per_lane=gpl, ms=ms)
+                     concurrent=best["concurrent"], block=block,
+                     groups_per_lane=gpl, ms=ms)
             trials.append(t)
             if log:
                 log(f"tune {t}")
@@ -88,7 +94,7 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
             for nb in sorted({1, level_best["batches"]}):
                 s.set_schedule(nb)
                 ms = _time_sweep(ev, s, reps)
-                t = dict(engine=engine, batches=nb,
+                t = dict(engine=engine, batches=nb, concurrent=False,
                          block=level_best["block"],
                          groups_per_lane=level_best["groups_per_lane"],
                          chunk_rounds=rounds, ms=ms)
@@ -108,7 +114,7 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
         for rule in priorities:
             s.set_priority(*rule)
             ms = _time_sweep(ev, s, reps)
-            t = dict(engine=ENGINE_QUEUE, batches=1,
+            t = dict(engine=ENGINE_QUEUE, batches=1, concurrent=False,
                      block=level_best["block"],
                      groups_per_lane=level_best["groups_per_lane"],
                      chunk_rounds=rounds, priority=tuple(rule), ms=ms)
@@ -120,6 +126,7 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
         s.set_priority(*best.get("priority", (-1, 0, 0)))
     s.set_engine(best["engine"], chunk_rounds=best.get("chunk_rounds", 0))
     s.set_schedule(best["batches"])
+    s.set_concurrent_batches(best["concurrent"])
     return best, trials
 
 

@@ -50,6 +50,10 @@ struct t2_sweeper {
     int intra = -1, inter = 0;       // priority schedule (intra < 0: levels)
     int64_t step_width = 0;
     bool use_graph = true;
+    bool concurrent = false;         // angle batches on parallel streams
+    std::vector<hipStream_t> side;   // their extra streams
+    std::vector<hipEvent_t> join_ev;
+    hipEvent_t fork_ev = nullptr;
     void *h_pin[2] = {nullptr, nullptr};   // pinned host staging (host link)
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
     t2::HostMesh hm;                 // kept for schedule rebuilds
@@ -213,6 +217,22 @@ void free_queue(t2_sweeper *s)
 }
 
 // ------------------------------------------------------------------ sweeps
+// Streams and events for angle batches swept concurrently (created once).
+int ensure_side_streams(t2_sweeper *s, int64_t n)
+{
+    if (!s->fork_ev)
+        T2_HIP(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming));
+    while (int64_t(s->side.size()) < n) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        T2_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        s->side.push_back(st);
+        T2_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        s->join_ev.push_back(ev);
+    }
+    return T2_OK;
+}
+
 template <typename T>
 int enqueue_sweep(t2_sweeper *s, const double *ang, const void *ang_t)
 {
@@ -233,17 +253,43 @@ int enqueue_sweep(t2_sweeper *s, const double *ang, const void *ang_t)
     a.lag_buf = static_cast<const T *>(s->d_lag_buf);
     a.solver = cell_solver(s->gauss);
     const auto *geo = static_cast<const t2::ItemGeo<T> *>(s->d_item_geo);
-    // per-step launches: the whole schedule (level engine) or the steps
-    // before the queued tail (hybrid)
-    const int64_t level_steps =
-        s->engine == T2_ENGINE_LEVELS ? s->sched.n_steps : s->queue_step;
-    for (int64_t st = 0; st < level_steps; st++) {
-        const int64_t off = s->sched.step_off[st];
-        a.geo = geo + off;
-        a.topo = s->d_item_topo + off;
-        a.n_items = s->sched.step_off[st + 1] - off;
-        T2_HIP(t2::launch_sweep_step<T>(a, int(s->ng), s->gpl, s->block,
-                                        s->stream));
+    auto launch_steps = [&](int64_t lo, int64_t hi, hipStream_t stream) {
+        for (int64_t st = lo; st < hi; st++) {
+            const int64_t off = s->sched.step_off[st];
+            a.geo = geo + off;
+            a.topo = s->d_item_topo + off;
+            a.n_items = s->sched.step_off[st + 1] - off;
+            T2_HIP(t2::launch_sweep_step<T>(a, int(s->ng), s->gpl, s->block,
+                                            stream));
+        }
+        return T2_OK;
+    };
+    const int64_t nb = int64_t(s->sched.batch_steps.size()) - 1;
+    if (s->engine == T2_ENGINE_LEVELS && s->concurrent && nb > 1) {
+        // angle batches are independent: batch b > 0 runs on its own
+        // stream, forked from and joined back to the handle's stream (in a
+        // captured graph: parallel branches), so one batch's launch gaps
+        // are filled by another's work
+        T2_TRY(ensure_side_streams(s, nb - 1));
+        T2_HIP(hipEventRecord(s->fork_ev, s->stream));
+        for (int64_t b = 1; b < nb; b++) {
+            hipStream_t sb = s->side[b - 1];
+            T2_HIP(hipStreamWaitEvent(sb, s->fork_ev, 0));
+            T2_TRY(launch_steps(s->sched.batch_steps[b],
+                                s->sched.batch_steps[b + 1], sb));
+            T2_HIP(hipEventRecord(s->join_ev[b - 1], sb));
+        }
+        T2_TRY(launch_steps(s->sched.batch_steps[0], s->sched.batch_steps[1],
+                            s->stream));
+        for (int64_t b = 1; b < nb; b++)
+            T2_HIP(hipStreamWaitEvent(s->stream, s->join_ev[b - 1], 0));
+    } else {
+        // per-step launches: the whole schedule (level engine) or the steps
+        // before the queued tail (hybrid)
+        T2_TRY(launch_steps(0,
+                            s->engine == T2_ENGINE_LEVELS ? s->sched.n_steps
+                                                          : s->queue_step,
+                            s->stream));
     }
     if (s->engine != T2_ENGINE_LEVELS && s->n_chunks > 0) {
         a.geo = geo;
@@ -460,6 +506,9 @@ int run_sweep(t2_sweeper *s, const double *ang, const void *ang_t = nullptr)
     T2_TRY(check_step_launches(s->engine == T2_ENGINE_LEVELS
                                    ? s->sched.n_steps
                                    : s->queue_step));
+    if (s->concurrent)   // created outside any stream capture
+        T2_TRY(ensure_side_streams(
+            s, int64_t(s->sched.batch_steps.size()) - 2));
     s->n_sweeps++;
     if (!s->use_graph)
         return enqueue_sweep_any(s, ang, ang_t);
@@ -1145,6 +1194,13 @@ int t2_sweeper_destroy(t2_sweeper *s)
         if (s->pin_ev[b])
             (void)hipEventDestroy(s->pin_ev[b]);
     }
+    for (size_t i = 0; i < s->side.size(); i++) {
+        (void)hipStreamSynchronize(s->side[i]);
+        (void)hipStreamDestroy(s->side[i]);
+        (void)hipEventDestroy(s->join_ev[i]);
+    }
+    if (s->fork_ev)
+        (void)hipEventDestroy(s->fork_ev);
     if (s->stream)
         (void)hipStreamDestroy(s->stream);
     delete s;
@@ -1172,6 +1228,7 @@ int t2_sweeper_info(const t2_sweeper *s, t2_info *info)
     info->intra_angle_p = s->intra;
     info->inter_angle_p = s->inter;
     info->max_cells_per_step = s->step_width;
+    info->concurrent_batches = s->concurrent ? 1 : 0;
     info->items_per_block = int32_t(s->block / (s->ng / s->gpl));
     info->groups_per_lane = s->gpl;
     info->chunk_rounds = s->chunk_rounds;
@@ -1247,6 +1304,18 @@ int t2_set_priority(t2_sweeper *s, int intra_angle_p, int inter_angle_p,
     s->inter = inter_angle_p;
     s->step_width = max_cells_per_step;
     return build_schedule_device(s);
+}
+
+int t2_set_concurrent_batches(t2_sweeper *s, int concurrent)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if ((concurrent != 0) == s->concurrent)
+        return T2_OK;
+    T2_HIP(hipStreamSynchronize(s->stream));
+    drop_graphs(s);
+    s->concurrent = concurrent != 0;
+    return T2_OK;
 }
 
 int t2_set_schedule(t2_sweeper *s, int angle_batches)

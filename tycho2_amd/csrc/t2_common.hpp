@@ -58,6 +58,7 @@ struct Schedule {
     std::vector<int64_t> step_off;     // [n_steps + 1]
     std::vector<uint32_t> work;        // [nc * na]
     std::vector<LaggedFace> lagged;    // sorted by (angle, cell, face)
+    std::vector<int64_t> batch_steps;  // angle batch b: steps [b], [b + 1]
 };
 
 // Levels of one angle; returns the lagged faces appended to `lagged`.

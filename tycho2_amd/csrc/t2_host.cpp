@@ -334,6 +334,7 @@ int build_schedule(const HostMesh &m, int64_t na, const double *omega,
     for (int b = 0; b < nb; b++)
         boff[b + 1] = boff[b] + blev[b];
     s.n_steps = boff[nb];
+    s.batch_steps = boff;
     s.min_levels = min_lev;
     s.step_off.assign(s.n_steps + 1, 0);
     for (int64_t a = 0; a < na; a++) {
@@ -697,6 +698,7 @@ int build_priority_schedule(const HostMesh &m, int64_t na,
         return fail(T2_ERR_ARG, "priority schedule: dependency graph left "
                                 "items unscheduled");
     s.n_steps = int64_t(s.step_off.size()) - 1;
+    s.batch_steps = {0, s.n_steps};
     return T2_OK;
 }
 

@@ -61,6 +61,8 @@ SIGNATURES = {
                                      ctypes.c_int]),
     "t2_set_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "t2_set_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "t2_set_concurrent_batches": (ctypes.c_int, [ctypes.c_void_p,
+                                                 ctypes.c_int]),
     "t2_set_priority": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int64]),
     "t2_set_engine": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
@@ -126,7 +128,8 @@ class T2Info(ctypes.Structure):
                 ("queue_first_step", ctypes.c_int32),
                 ("intra_angle_p", ctypes.c_int32),
                 ("inter_angle_p", ctypes.c_int32),
-                ("max_cells_per_step", ctypes.c_int64)]
+                ("max_cells_per_step", ctypes.c_int64),
+                ("concurrent_batches", ctypes.c_int32)]
 
 
 class T2RefineInfo(ctypes.Structure):
@@ -303,6 +306,11 @@ class SweeperHIP:
     def set_order(self, order: int) -> None:
         """ORDER_CELL or ORDER_FACES: item order within a step."""
         _check(lib().t2_set_order(self.handle, int(order)))
+
+    def set_concurrent_batches(self, on: bool) -> None:
+        """Level engine: angle batches on parallel streams (True) or one
+        after the other (False, default)."""
+        _check(lib().t2_set_concurrent_batches(self.handle, int(bool(on))))
 
     def set_priority(self, intra_angle_p: int, inter_angle_p: int = 0,
                      max_cells_per_step: int = 0) -> None:
