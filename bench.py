@@ -236,14 +236,16 @@ def batch_leg(args, ev):
                                             engines=engines, log=log)
             out = {}
             for t in trials:
-                name = ("levels" if t["engine"] == ENGINE_LEVELS
-                        else f"{prefix}_r{t['chunk_rounds']}")
+                if t["engine"] == ENGINE_LEVELS:
+                    name = "levels_conc" if t["concurrent"] else "levels"
+                else:
+                    name = f"{prefix}_r{t['chunk_rounds']}"
                 out[name] = t["ms"]
                 log(f"wave {w}: {name} {t['ms']:.2f} ms/sweep, "
                     f"{solves / t['ms'] / 1e6:.1f} G solves/s")
             return out
 
-        timings = {"levels": float("inf")}
+        timings = {"levels": float("inf"), "levels_conc": float("inf")}
         with SweepBatch(hs) as b:
             timings.update(tuned(b, "queue", (ENGINE_LEVELS, ENGINE_QUEUE)))
         if rule:
@@ -267,12 +269,17 @@ def batch_leg(args, ev):
         if rule:
             tot["queue_prio"] = tot.get("queue_prio", 0.0) + min(
                 timings[k] for k in timings if k.startswith("queue_prio"))
+        tot["levels_conc"] = tot.get("levels_conc", 0.0) + \
+            timings["levels_conc"]
         tot["best"] += timings[best]
         tot["solves"] += solves
         with SweepBatch(hs) as b:
-            b.set_engine(ENGINE_LEVELS if best == "levels" else ENGINE_QUEUE,
-                         0, 0 if best == "levels"
-                         else int(best.rsplit("_r", 1)[1]))
+            if best.startswith("levels"):
+                b.set_engine(ENGINE_LEVELS, 0, 0)
+                b.set_concurrent(best == "levels_conc")
+            else:
+                b.set_engine(ENGINE_QUEUE, 0,
+                             int(best.rsplit("_r", 1)[1]))
             t2 = time.time()
             its, sec = b.solve_krylov(300, 1e-10, 20)
             rec["solve_s"] = round(time.time() - t2, 3)
@@ -304,6 +311,8 @@ def batch_leg(args, ev):
                                    f"maxCellsPerStep = {list(rule)}")},
            "levels_solves_per_s": round(tot["solves"] /
                                         (tot["levels"] * 1e-3)),
+           "levels_concurrent_solves_per_s": round(
+               tot["solves"] / (tot["levels_conc"] * 1e-3)),
            "queue_solves_per_s": round(tot["solves"] / (tot["queue"] * 1e-3)),
            "queue_priority_solves_per_s": (
                round(tot["solves"] / (tot["queue_prio"] * 1e-3))

@@ -304,7 +304,8 @@ def batch_members(precision, groups=(4, 4, 4)):
 
 @pytest.mark.parametrize("precision,engine,groups", [
     (FP64, 0, (4, 4, 4)), (FP32, 0, (4, 4, 4)), (FP64, 0, (1, 16, 4)),
-    (FP64, 1, (1, 16, 4)), (FP32, 1, (32, 2, 4)), (FP64, 1, (64, 8, 1))])
+    (FP64, 1, (1, 16, 4)), (FP32, 1, (32, 2, 4)), (FP64, 1, (64, 8, 1)),
+    (FP64, 2, (1, 16, 4)), (FP32, 2, (32, 2, 32))])
 def test_batch_sweep_bitwise(precision, engine, groups):
     """A batch (merged per-step launches, or one persistent ready-queue
     launch for all members) gives every member what its own source
@@ -319,7 +320,9 @@ def test_batch_sweep_bitwise(precision, engine, groups):
         for _ in range(3):
             s.sweep_step()
     with SweepBatch(mem) as b:
-        b.set_engine(engine, 1 << 20)
+        # engine 2 here: the level form with group counts on parallel streams
+        b.set_engine(engine if engine < 2 else 0, 1 << 20)
+        b.set_concurrent(engine == 2)
         inf = b.info()
         assert inf["n_steps"] == max(s.info()["n_steps"] for s in mem)
         for _ in range(3):
@@ -465,7 +468,9 @@ def test_batch_autotune_keeps_results():
         best, trials = autotune.tune_batch(b, reps=1)
         assert best == min(trials, key=lambda t: t["ms"])
         assert {t["engine"] for t in trials} == {0, 1}
-        assert len(trials) == 1 + len(autotune.CHUNK_ROUNDS)
+        assert len(trials) == 2 + len(autotune.CHUNK_ROUNDS)
+        assert {t["concurrent"] for t in trials if t["engine"] == 0} == \
+            {False, True}
         for m in mem:
             m.set_phi(np.zeros_like(m.phi()))
         b.sweep_step()

@@ -132,16 +132,20 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
 
 def tune_batch(b, reps: int = 3, rounds=CHUNK_ROUNDS,
                engines=(ENGINE_LEVELS, ENGINE_QUEUE), log=None):
-    """Tune batch `b` in place over its engine (merged per-step launches or
-    one persistent ready-queue launch) and the queue's chunk size; returns
+    """Tune batch `b` in place over its engine (merged per-step launches,
+    their group counts interleaved or on parallel streams, or one
+    persistent ready-queue launch) and the queue's chunk size; returns
     (best, trials) like tune().  A level form that refuses the members'
     schedules (too many steps) is skipped."""
     from .sweeper import T2Error
     ev = HipEvents()
     trials = []
     for engine in engines:
-        for r in (rounds if engine == ENGINE_QUEUE else (0,)):
+        variants = ([(r, False) for r in rounds] if engine == ENGINE_QUEUE
+                    else [(0, False), (0, True)])
+        for r, conc in variants:
             b.set_engine(engine, 0, r)
+            b.set_concurrent(conc)
             try:
                 b.sweep_only()
             except T2Error as e:
@@ -150,7 +154,7 @@ def tune_batch(b, reps: int = 3, rounds=CHUNK_ROUNDS,
                 continue
             b.synchronize()
             ms = time_region(ev, b, b.sweep_only, reps) / reps
-            t = dict(engine=engine, chunk_rounds=r, ms=ms)
+            t = dict(engine=engine, chunk_rounds=r, concurrent=conc, ms=ms)
             trials.append(t)
             if log:
                 log(f"tune batch {t}")
@@ -158,4 +162,5 @@ def tune_batch(b, reps: int = 3, rounds=CHUNK_ROUNDS,
         raise RuntimeError("no batch engine could sweep these members")
     best = min(trials, key=lambda t: t["ms"])
     b.set_engine(best["engine"], 0, best["chunk_rounds"])
+    b.set_concurrent(best["concurrent"])
     return best, trials

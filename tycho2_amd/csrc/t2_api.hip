@@ -1735,6 +1735,10 @@ struct t2_batch {
     hipStream_t stream = nullptr;
     hipEvent_t order = nullptr;          // orders member and batch streams
     int engine = T2_ENGINE_LEVELS;
+    bool concurrent = false;             // level form: group counts in parallel
+    std::vector<hipStream_t> side;       // their extra streams
+    std::vector<hipEvent_t> join_ev;
+    hipEvent_t fork_ev = nullptr;
     int chunk_rounds = 4;
     int64_t spin_limit = 0;
     int64_t n_steps = 0, n_items = 0;
@@ -2077,6 +2081,36 @@ int batch_enqueue(t2_batch *b)
             b->d_chunks, q, b->grid, b->stream));
         return T2_OK;
     }
+    auto launch = [&](LevelGroup &g, int64_t st, hipStream_t stream) {
+        t2::SweepArgs<T> a{};
+        a.problems = static_cast<const t2::ProblemDesc<T> *>(g.d_desc);
+        a.active = g.d_active;
+        a.geo = static_cast<const t2::ItemGeo<T> *>(g.d_geo) + g.step_off[st];
+        a.topo = g.d_topo + g.step_off[st];
+        a.n_items = g.step_off[st + 1] - g.step_off[st];
+        T2_HIP(t2::launch_sweep_step<T>(a, g.ng, 1, t2::kMaxBlock, stream));
+        return T2_OK;
+    };
+    const size_t ng = b->groups.size();
+    if (b->concurrent && ng > 1) {
+        // members of different group counts share nothing: group g > 0
+        // runs its merged steps on a stream of its own (a parallel branch
+        // of the captured graph), forked from and joined to the batch's
+        T2_HIP(hipEventRecord(b->fork_ev, b->stream));
+        for (size_t i = 1; i < ng; i++) {
+            LevelGroup &g = b->groups[i];
+            T2_HIP(hipStreamWaitEvent(b->side[i - 1], b->fork_ev, 0));
+            for (int64_t st = 0; st + 1 < int64_t(g.step_off.size()); st++)
+                T2_TRY(launch(g, st, b->side[i - 1]));
+            T2_HIP(hipEventRecord(b->join_ev[i - 1], b->side[i - 1]));
+        }
+        LevelGroup &g0 = b->groups[0];
+        for (int64_t st = 0; st + 1 < int64_t(g0.step_off.size()); st++)
+            T2_TRY(launch(g0, st, b->stream));
+        for (size_t i = 1; i < ng; i++)
+            T2_HIP(hipStreamWaitEvent(b->stream, b->join_ev[i - 1], 0));
+        return T2_OK;
+    }
     int64_t longest = 0;
     for (auto &g : b->groups)
         longest = std::max<int64_t>(longest, int64_t(g.step_off.size()) - 1);
@@ -2084,15 +2118,7 @@ int batch_enqueue(t2_batch *b)
         for (auto &g : b->groups) {
             if (st + 1 >= int64_t(g.step_off.size()))
                 continue;
-            t2::SweepArgs<T> a{};
-            a.problems = static_cast<const t2::ProblemDesc<T> *>(g.d_desc);
-            a.active = g.d_active;
-            a.geo = static_cast<const t2::ItemGeo<T> *>(g.d_geo) +
-                    g.step_off[st];
-            a.topo = g.d_topo + g.step_off[st];
-            a.n_items = g.step_off[st + 1] - g.step_off[st];
-            T2_HIP(t2::launch_sweep_step<T>(a, g.ng, 1, t2::kMaxBlock,
-                                            b->stream));
+            T2_TRY(launch(g, st, b->stream));
         }
     return T2_OK;
 }
@@ -2169,6 +2195,20 @@ int batch_sweep(t2_batch *b)
         }
         T2_HIP(hipStreamSynchronize(b->stream));
         b->uploaded = b->active;
+    }
+    if (b->concurrent && b->engine == T2_ENGINE_LEVELS) {
+        // streams and events of the parallel branches, outside any capture
+        if (!b->fork_ev)
+            T2_HIP(hipEventCreateWithFlags(&b->fork_ev,
+                                           hipEventDisableTiming));
+        while (b->side.size() + 1 < b->groups.size()) {
+            hipStream_t st = nullptr;
+            hipEvent_t ev = nullptr;
+            T2_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            b->side.push_back(st);
+            T2_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            b->join_ev.push_back(ev);
+        }
     }
     if (!b->graph) {
         hipGraph_t g = nullptr;
@@ -2291,9 +2331,30 @@ int t2_batch_destroy(t2_batch *b)
         (void)hipFree(b->d_active);
     if (b->order)
         (void)hipEventDestroy(b->order);
+    for (size_t i = 0; i < b->side.size(); i++) {
+        (void)hipStreamSynchronize(b->side[i]);
+        (void)hipStreamDestroy(b->side[i]);
+        (void)hipEventDestroy(b->join_ev[i]);
+    }
+    if (b->fork_ev)
+        (void)hipEventDestroy(b->fork_ev);
     if (b->stream)
         (void)hipStreamDestroy(b->stream);
     delete b;
+    return T2_OK;
+}
+
+int t2_batch_set_concurrent(t2_batch *b, int concurrent)
+{
+    t2::clear_error();
+    if (!b)
+        return fail(T2_ERR_ARG, "null batch");
+    if ((concurrent != 0) == b->concurrent)
+        return T2_OK;
+    T2_HIP(hipSetDevice(b->device));
+    T2_HIP(hipStreamSynchronize(b->stream));
+    drop_batch_graph(b);
+    b->concurrent = concurrent != 0;
     return T2_OK;
 }
 

@@ -96,6 +96,8 @@ SIGNATURES = {
     "t2_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "t2_batch_set_engine": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int64]),
+    "t2_batch_set_concurrent": (ctypes.c_int, [ctypes.c_void_p,
+                                               ctypes.c_int]),
     "t2_batch_sweep_step": (ctypes.c_int, [ctypes.c_void_p]),
     "t2_batch_sweep_only": (ctypes.c_int, [ctypes.c_void_p]),
     "t2_batch_info": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip]),
@@ -468,6 +470,11 @@ class SweepBatch:
                    chunk_rounds: int = 0) -> None:
         _check(lib().t2_batch_set_engine(self.handle, int(engine),
                                          int(chunk_rounds), int(spin_limit)))
+
+    def set_concurrent(self, on: bool) -> None:
+        """Level form: members of different group counts on parallel
+        streams (True) or interleaved on one stream (False, default)."""
+        _check(lib().t2_batch_set_concurrent(self.handle, int(bool(on))))
 
     def sweep_step(self) -> None:
         _check(lib().t2_batch_sweep_step(self.handle))
