@@ -237,7 +237,8 @@ def batch_leg(args, ev):
             out = {}
             for t in trials:
                 if t["engine"] == ENGINE_LEVELS:
-                    name = "levels_conc" if t["concurrent"] else "levels"
+                    name = (f"levels_c{t['concurrent']}" if t["concurrent"]
+                            else "levels")
                 else:
                     name = f"{prefix}_r{t['chunk_rounds']}"
                 out[name] = t["ms"]
@@ -245,7 +246,7 @@ def batch_leg(args, ev):
                     f"{solves / t['ms'] / 1e6:.1f} G solves/s")
             return out
 
-        timings = {"levels": float("inf"), "levels_conc": float("inf")}
+        timings = {"levels": float("inf")}
         with SweepBatch(hs) as b:
             timings.update(tuned(b, "queue", (ENGINE_LEVELS, ENGINE_QUEUE)))
         if rule:
@@ -269,14 +270,16 @@ def batch_leg(args, ev):
         if rule:
             tot["queue_prio"] = tot.get("queue_prio", 0.0) + min(
                 timings[k] for k in timings if k.startswith("queue_prio"))
-        tot["levels_conc"] = tot.get("levels_conc", 0.0) + \
-            timings["levels_conc"]
+        tot["levels_conc"] = tot.get("levels_conc", 0.0) + min(
+            [v for k, v in timings.items() if k.startswith("levels_c")] +
+            [float("inf")])
         tot["best"] += timings[best]
         tot["solves"] += solves
         with SweepBatch(hs) as b:
             if best.startswith("levels"):
                 b.set_engine(ENGINE_LEVELS, 0, 0)
-                b.set_concurrent(best == "levels_conc")
+                b.set_concurrent(0 if best == "levels"
+                                 else int(best[len("levels_c"):]))
             else:
                 b.set_engine(ENGINE_QUEUE, 0,
                              int(best.rsplit("_r", 1)[1]))

@@ -309,10 +309,12 @@ int t2_batch_create(t2_batch **out, t2_sweeper *const *members, int count);
 int t2_batch_destroy(t2_batch *b);
 int t2_batch_set_engine(t2_batch *b, int engine, int chunk_rounds,
                         int64_t spin_limit);
-/* Level form: members of different group counts (independent merged
- * schedules) swept on parallel streams, parallel branches of the captured
- * graph (1), or their launches interleaved on one stream (0, default). */
-int t2_batch_set_concurrent(t2_batch *b, int concurrent);
+/* Level form: `chains` >= 1 sweeps independent merged schedules on
+ * parallel streams (parallel branches of the captured graph): one per group
+ * count, or, for chains > 1, up to `chains` per group count with that group
+ * count's members dealt out by work; 0 (default) interleaves every launch
+ * on one stream.  Results do not depend on it. */
+int t2_batch_set_concurrent(t2_batch *b, int chains);
 /* t2_sweep_step / t2_sweep_only of every member in one batched sweep */
 int t2_batch_sweep_step(t2_batch *b);
 int t2_batch_sweep_only(t2_batch *b);

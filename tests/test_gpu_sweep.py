@@ -305,7 +305,8 @@ def batch_members(precision, groups=(4, 4, 4)):
 @pytest.mark.parametrize("precision,engine,groups", [
     (FP64, 0, (4, 4, 4)), (FP32, 0, (4, 4, 4)), (FP64, 0, (1, 16, 4)),
     (FP64, 1, (1, 16, 4)), (FP32, 1, (32, 2, 4)), (FP64, 1, (64, 8, 1)),
-    (FP64, 2, (1, 16, 4)), (FP32, 2, (32, 2, 32))])
+    (FP64, 2, (1, 16, 4)), (FP32, 2, (32, 2, 32)), (FP64, 3, (4, 4, 4)),
+    (FP32, 3, (16, 4, 16))])
 def test_batch_sweep_bitwise(precision, engine, groups):
     """A batch (merged per-step launches, or one persistent ready-queue
     launch for all members) gives every member what its own source
@@ -320,9 +321,10 @@ def test_batch_sweep_bitwise(precision, engine, groups):
         for _ in range(3):
             s.sweep_step()
     with SweepBatch(mem) as b:
-        # engine 2 here: the level form with group counts on parallel streams
+        # engine 2 / 3 here: the level form on parallel streams, one chain
+        # per group count / up to two chains per group count
         b.set_engine(engine if engine < 2 else 0, 1 << 20)
-        b.set_concurrent(engine == 2)
+        b.set_concurrent({2: 1, 3: 2}.get(engine, 0))
         inf = b.info()
         assert inf["n_steps"] == max(s.info()["n_steps"] for s in mem)
         for _ in range(3):
@@ -468,9 +470,10 @@ def test_batch_autotune_keeps_results():
         best, trials = autotune.tune_batch(b, reps=1)
         assert best == min(trials, key=lambda t: t["ms"])
         assert {t["engine"] for t in trials} == {0, 1}
-        assert len(trials) == 2 + len(autotune.CHUNK_ROUNDS)
+        assert len(trials) == len(autotune.CHAINS) + len(
+            autotune.CHUNK_ROUNDS)
         assert {t["concurrent"] for t in trials if t["engine"] == 0} == \
-            {False, True}
+            set(autotune.CHAINS)
         for m in mem:
             m.set_phi(np.zeros_like(m.phi()))
         b.sweep_step()

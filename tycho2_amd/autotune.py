@@ -24,6 +24,7 @@ BATCHES = (1, 2, 4, 8, 16)
 CHUNK_ROUNDS = (2, 4, 8)          # ready-queue chunk size, in rounds
 BLOCKS = (256, 128, 64)          # threads per block
 GROUPS_PER_LANE = (1, 2)
+CHAINS = (0, 1, 2, 4)             # batch level form: parallel chains
 
 
 def _time_sweep(ev, s, reps):
@@ -141,8 +142,8 @@ def tune_batch(b, reps: int = 3, rounds=CHUNK_ROUNDS,
     ev = HipEvents()
     trials = []
     for engine in engines:
-        variants = ([(r, False) for r in rounds] if engine == ENGINE_QUEUE
-                    else [(0, False), (0, True)])
+        variants = ([(r, 0) for r in rounds] if engine == ENGINE_QUEUE
+                    else [(0, c) for c in CHAINS])
         for r, conc in variants:
             b.set_engine(engine, 0, r)
             b.set_concurrent(conc)

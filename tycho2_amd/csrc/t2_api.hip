@@ -1735,7 +1735,8 @@ struct t2_batch {
     hipStream_t stream = nullptr;
     hipEvent_t order = nullptr;          // orders member and batch streams
     int engine = T2_ENGINE_LEVELS;
-    bool concurrent = false;             // level form: group counts in parallel
+    int concurrent = 0;                  // level form: chains per group count
+                                         // (0: all launches on one stream)
     std::vector<hipStream_t> side;       // their extra streams
     std::vector<hipEvent_t> join_ev;
     hipEvent_t fork_ev = nullptr;
@@ -1978,13 +1979,34 @@ int build_levels(t2_batch *b)
         if (std::find(ngs.begin(), ngs.end(), int(m->ng)) == ngs.end())
             ngs.push_back(int(m->ng));
     std::sort(ngs.begin(), ngs.end());
+    // one merged schedule per group count, or, for parallel chains, up to
+    // `concurrent` per group count with the members dealt out by work
+    // (largest first, each to the lightest chain)
+    const int chains = std::max(1, b->concurrent);
     for (int ng : ngs) {
-        LevelGroup g;
-        g.ng = ng;
+        std::vector<int> idx;
         for (size_t i = 0; i < b->members.size(); i++)
             if (b->members[i]->ng == ng)
-                g.idx.push_back(int(i));
-        b->groups.push_back(g);
+                idx.push_back(int(i));
+        auto work = [&](int i) {
+            return b->members[i]->nc * b->members[i]->na * b->members[i]->ng;
+        };
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](int x, int y) { return work(x) > work(y); });
+        const size_t nch = std::min<size_t>(size_t(chains), idx.size());
+        std::vector<LevelGroup> part(nch);
+        std::vector<int64_t> load(nch, 0);
+        for (int i : idx) {
+            const size_t c = size_t(
+                std::min_element(load.begin(), load.end()) - load.begin());
+            part[c].idx.push_back(i);
+            load[c] += work(i);
+        }
+        for (auto &g : part) {
+            g.ng = ng;
+            std::sort(g.idx.begin(), g.idx.end());
+            b->groups.push_back(g);
+        }
     }
     for (auto &g : b->groups)
         T2_TRY(b->precision == T2_FP64 ? build_level_group<double>(b, g)
@@ -2092,10 +2114,10 @@ int batch_enqueue(t2_batch *b)
         return T2_OK;
     };
     const size_t ng = b->groups.size();
-    if (b->concurrent && ng > 1) {
-        // members of different group counts share nothing: group g > 0
-        // runs its merged steps on a stream of its own (a parallel branch
-        // of the captured graph), forked from and joined to the batch's
+    if (b->concurrent > 0 && ng > 1) {
+        // merged schedules share nothing: group g > 0 runs its merged steps
+        // on a stream of its own (a parallel branch of the captured graph),
+        // forked from and joined to the batch's
         T2_HIP(hipEventRecord(b->fork_ev, b->stream));
         for (size_t i = 1; i < ng; i++) {
             LevelGroup &g = b->groups[i];
@@ -2196,7 +2218,7 @@ int batch_sweep(t2_batch *b)
         T2_HIP(hipStreamSynchronize(b->stream));
         b->uploaded = b->active;
     }
-    if (b->concurrent && b->engine == T2_ENGINE_LEVELS) {
+    if (b->concurrent > 0 && b->engine == T2_ENGINE_LEVELS) {
         // streams and events of the parallel branches, outside any capture
         if (!b->fork_ev)
             T2_HIP(hipEventCreateWithFlags(&b->fork_ev,
@@ -2344,17 +2366,24 @@ int t2_batch_destroy(t2_batch *b)
     return T2_OK;
 }
 
-int t2_batch_set_concurrent(t2_batch *b, int concurrent)
+int t2_batch_set_concurrent(t2_batch *b, int chains)
 {
     t2::clear_error();
     if (!b)
         return fail(T2_ERR_ARG, "null batch");
-    if ((concurrent != 0) == b->concurrent)
+    if (chains < 0 || chains > 64)
+        return fail(T2_ERR_ARG, "chains must be in [0, 64]");
+    if (chains == b->concurrent)
         return T2_OK;
     T2_HIP(hipSetDevice(b->device));
     T2_HIP(hipStreamSynchronize(b->stream));
     drop_batch_graph(b);
-    b->concurrent = concurrent != 0;
+    // the member split of the merged schedules follows the chain count
+    if (std::max(1, chains) != std::max(1, b->concurrent)) {
+        free_levels(b);
+        b->levels_built = false;
+    }
+    b->concurrent = chains;
     return T2_OK;
 }
 

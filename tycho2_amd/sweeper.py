@@ -471,10 +471,11 @@ class SweepBatch:
         _check(lib().t2_batch_set_engine(self.handle, int(engine),
                                          int(chunk_rounds), int(spin_limit)))
 
-    def set_concurrent(self, on: bool) -> None:
-        """Level form: members of different group counts on parallel
-        streams (True) or interleaved on one stream (False, default)."""
-        _check(lib().t2_batch_set_concurrent(self.handle, int(bool(on))))
+    def set_concurrent(self, chains: int) -> None:
+        """Level form: independent merged schedules on parallel streams,
+        up to `chains` per group count (1: one per group count), or 0 to
+        interleave every launch on one stream (default)."""
+        _check(lib().t2_batch_set_concurrent(self.handle, int(chains)))
 
     def sweep_step(self) -> None:
         _check(lib().t2_batch_sweep_step(self.handle))
