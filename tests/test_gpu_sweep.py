@@ -339,7 +339,8 @@ def test_batch_sweep_bitwise(precision, engine, groups):
 
 @pytest.mark.parametrize("precision,engine,method", [
     (FP64, 0, "krylov"), (FP64, 1, "krylov"), (FP32, 1, "krylov"),
-    (FP64, 1, "si"), (FP64, 0, "si")])
+    (FP64, 1, "si"), (FP64, 0, "si"), (FP64, 2, "krylov"),
+    (FP64, 2, "si")])
 def test_batch_solves_bitwise(precision, engine, method):
     """Lock-step batched solves give every member its own solve's phi and
     psi bit for bit, and its own iteration count."""
@@ -356,7 +357,9 @@ def test_batch_solves_bitwise(precision, engine, method):
         else:
             counts.append(len(s.solve_source_iteration(300, 1e-10)))
     with SweepBatch(mem) as b:
-        b.set_engine(engine, 1 << 20)
+        # engine 2 here: the level form, group counts on parallel streams
+        b.set_engine(engine if engine < 2 else 0, 1 << 20)
+        b.set_concurrent(2 if engine == 2 else 0)
         if method == "krylov":
             its, sec = b.solve_krylov(100, 1e-10, 10)
         else:
