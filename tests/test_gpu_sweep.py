@@ -485,3 +485,31 @@ def test_batch_autotune_keeps_results():
         assert np.array_equal(s.phi(), m.phi())
     for s in own + mem:
         s.close()
+
+
+def test_refined_on_concurrent_paths_bitwise():
+    """The refined solve on a handle with two concurrent angle batches, and
+    the lock-step refined solve on a batch with two chains per group count,
+    equal the plain one-stream solves bit for bit."""
+    from tycho2_amd.sweeper import SweepBatch
+    print("refined on concurrent paths", flush=True)
+    groups = (4, 16, 1)
+    own = batch_members(FP32, groups)
+    ref = [s.solve_refined(100, 1e-6, 1e-5, 10)[0] for s in own]
+    phis = [s.phi() for s in own]
+    conc = batch_members(FP32, groups)
+    for s, info, phi in zip(conc, ref, phis):
+        s.set_schedule(2)
+        s.set_concurrent_batches(True)
+        got, _ = s.solve_refined(100, 1e-6, 1e-5, 10)
+        assert got == info
+        assert np.array_equal(s.phi(), phi)
+    mem = batch_members(FP32, groups)
+    with SweepBatch(mem) as b:
+        b.set_concurrent(2)
+        infos, _ = b.solve_refined(100, 1e-6, 1e-5, 10)
+    assert infos == ref
+    for m, phi in zip(mem, phis):
+        assert np.array_equal(m.phi(), phi)
+    for s in own + conc + mem:
+        s.close()
