@@ -128,6 +128,13 @@ def test_null_handle_errors():
     info = sweeper.T2Info()
     assert lib.t2_sweeper_info(None, ctypes.byref(info)) == -1
     assert lib.t2_sweeper_destroy(None) == 0
+    # knobs and batch calls added with the schedules and concurrency
+    assert lib.t2_set_priority(None, 1, 0, 0) == -1
+    assert lib.t2_set_concurrent_batches(None, 1) == -1
+    assert lib.t2_batch_set_concurrent(None, 1) == -1
+    assert lib.t2_batch_solve_refined(None, 10, 1e-6, 1e-5, 10, None,
+                                      None) == -1
+    assert lib.t2_batch_destroy(None) == 0
 
 
 def test_geometry_rejects_bad_face_map():
