@@ -151,6 +151,22 @@ def sweep(mesh: Mesh, omega, ng, sigma_t, source, psi=None, method=NOPIVOT,
     return psi, lag.value
 
 
+def defect(mesh: Mesh, omega, ng, sigma_t, source, psi):
+    """Nodal density d[c][a][v][g] of the defect r = source - L psi (weak
+    form V/20 (I + J) d = r), every interior face reading the current psi."""
+    om = _f64(omega)
+    na = om.shape[0]
+    st = _sig(sigma_t, mesh.nc, ng)
+    src = _f64(source).reshape(mesh.nc, na, 4, ng)
+    ps = _f64(psi).reshape(mesh.nc, na, 4, ng)
+    out = np.empty((mesh.nc, na, 4, ng))
+    m = mesh.cstruct()
+    lib().or_defect(ctypes.byref(m), ctypes.c_int64(na), _p(om, "d"),
+                    ctypes.c_int64(ng), _p(st, "d"), _p(src, "d"),
+                    _p(ps, "d"), _p(out, "d"))
+    return out
+
+
 def psi_to_phi(psi, wt):
     nc, na, _, ng = psi.shape
     phi = np.empty((nc, 4, ng))

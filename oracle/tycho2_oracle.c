@@ -611,6 +611,59 @@ int or_sweep(const or_mesh *m, int64_t na, const double *omega, int64_t ng,
                            nthreads, 0, na, n_lagged);
 }
 
+/* ---------------------------------------------------------------- defect */
+/* Defect r = b - M psi_c of every (cell, angle, group), with M and b the
+ * cell system above (Transport.cc:54-271, applied instead of inverted) and
+ * every interior incoming face reading the upwind cell's psi (no lagging:
+ * the full operator).  Stored as the nodal density d = (20/V)(r - sum(r)/5),
+ * whose weak form V/20 (I + J) d is r, so that a sweep with d as its source
+ * gives the correction psi* - psi.  source, psi, density: [c][a][v][g]. */
+void or_defect(const or_mesh *m, int64_t na, const double *omega, int64_t ng,
+               const double *sigma_t, const double *source, const double *psi,
+               double *density)
+{
+    for (int64_t c = 0; c < m->nc; c++)
+        for (int64_t a = 0; a < na; a++) {
+            const double *om = omega + 3 * a;
+            double abar[4], odn[4];
+            for (int k = 0; k < 4; k++) {
+                odn[k] = omega_dot_n(om, m->normal + (4 * c + k) * 3);
+                abar[k] = m->area[4 * c + k] * odn[k];
+            }
+            const double vol = m->volume[c];
+            for (int64_t g = 0; g < ng; g++) {
+                double q[4], up[4][4], mat[4][4], b[4], r[4];
+                for (int v = 0; v < 4; v++)
+                    q[v] = source[((c * na + a) * 4 + v) * ng + g];
+                for (int k = 0; k < 4; k++) {
+                    for (int j = 0; j < 4; j++)
+                        up[k][j] = 0.0;
+                    int64_t nb = m->adj[4 * c + k];
+                    if (odn[k] > 0 || nb < 0)
+                        continue;
+                    for (int j = 0; j < 4; j++) {
+                        if (j == k)
+                            continue;
+                        int64_t fv = m->c2f[(4 * c + k) * 4 + j];
+                        int64_t nv = m->nbrv[(4 * c + k) * 3 + fv];
+                        up[k][j] = psi[((nb * na + a) * 4 + nv) * ng + g];
+                    }
+                }
+                cell_system(vol, abar, sigma_t[c * ng + g], q, up, mat, b);
+                double sum = 0.0;
+                for (int i = 0; i < 4; i++) {
+                    r[i] = b[i];
+                    for (int j = 0; j < 4; j++)
+                        r[i] -= mat[i][j] * psi[((c * na + a) * 4 + j) * ng + g];
+                    sum += r[i];
+                }
+                for (int v = 0; v < 4; v++)
+                    density[((c * na + a) * 4 + v) * ng + g] =
+                        (r[v] - 0.2 * sum) / (vol / 20.0);
+            }
+        }
+}
+
 /* --------------------------------------------------------------- moments */
 /* Util.cc:85-97: angle-ascending accumulation for each (cell, vertex, group) */
 void or_psi_to_phi(int64_t nc, int64_t na, int64_t ng, const double *wt,

@@ -76,6 +76,13 @@ int or_sweep_angles(const or_mesh *m, int64_t na, const double *omega,
                     double *psi, int method, int nthreads, int64_t a0,
                     int64_t a1, int64_t *n_lagged);
 
+/* Defect r = q - L psi of every (cell, angle, group) for a given psi and
+ * total source, as the nodal density whose weak form is r (the full
+ * operator: no face is lagged).  source, psi, density: [c][a][v][g]. */
+void or_defect(const or_mesh *m, int64_t na, const double *omega, int64_t ng,
+               const double *sigma_t, const double *source, const double *psi,
+               double *density);
+
 void or_psi_to_phi(int64_t nc, int64_t na, int64_t ng, const double *wt,
                    const double *psi, double *phi);
 

@@ -82,9 +82,9 @@ def local_system(P, omega, sigma, q_nodes, upwind):
     return M, rhs
 
 
-def global_sweep_solution(coords, cell_nodes, adj, omega, sigma, q_nodes):
-    """psi[c][v] for one direction by a dense solve of the global system
-    (vacuum inflow on the boundary)."""
+def global_system(coords, cell_nodes, adj, omega, sigma, q_nodes):
+    """(A, b) of the global DG system A psi = b of one direction, psi
+    ordered [c][v] (vacuum inflow on the boundary)."""
     nc = cell_nodes.shape[0]
     A = np.zeros((4 * nc, 4 * nc))
     b = np.zeros(4 * nc)
@@ -111,4 +111,10 @@ def global_sweep_solution(coords, cell_nodes, adj, omega, sigma, q_nodes):
                 for qp in range(3):
                     A[4 * c:4 * c + 4, 4 * u + vu] += \
                         on * fw * flam[qp] * flam[qp][j]
-    return np.linalg.solve(A, b).reshape(nc, 4)
+    return A, b
+
+
+def global_sweep_solution(coords, cell_nodes, adj, omega, sigma, q_nodes):
+    """psi[c][v] for one direction by a dense solve of the global system."""
+    A, b = global_system(coords, cell_nodes, adj, omega, sigma, q_nodes)
+    return np.linalg.solve(A, b).reshape(cell_nodes.shape[0], 4)
