@@ -279,6 +279,21 @@ int t2_solve_refined(t2_sweeper *s, int64_t iter_max, double rtol_sweep,
                      double rtol_correction, int64_t restart,
                      t2_refine_info *info, double *phi1);
 
+/* FP64 defect of the transport equations for an angular flux: the operator
+ * apply of step 2 above, on its own.  r = q + M S phi - L psi for every
+ * (cell, angle, group), with q the handle's source, phi its scalar flux
+ * (t2_set_phi or the last solve) and L the cell systems of
+ * src/Transport.cc:54-271 (src/Transport.cc:687-733 per group) applied, not
+ * inverted, with every incoming interior face reading the upwind cell's psi
+ * (no face lagged).  The reference forms no defect; its sweep
+ * (src/SweeperAbstract.hh:50-52) inverts this operator.
+ * psi: host [c][a][v][g] (src/PsiData.hh:162), stored as the handle's
+ * angular flux first (rounded to FP32 on FP32 handles), or NULL for the
+ * handle's own.  density: host [c][a][v][g], the nodal source density d with
+ * weak form V/20 (I + J) d = r on each cell, so that a sweep with source d
+ * gives L^-1 r; rounded to FP32 on FP32 handles. */
+int t2_defect(t2_sweeper *s, const double *psi, double *density);
+
 /* One sweep with the current scalar flux: psi = L^-1 (q + S phi) and
  * phi <- D psi (one source-iteration step; the benchmark's unit of work). */
 int t2_sweep_step(t2_sweeper *s);

@@ -1599,6 +1599,74 @@ int t2_get_psi(const t2_sweeper *s_const, double *psi)
     return download_host(s, psi, s->d_stage, sizeof(double) * n);
 }
 
+int t2_defect(t2_sweeper *s, const double *psi, double *density)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (!density)
+        return fail(T2_ERR_ARG, "t2_defect: null density");
+    const int64_t n = s->n_psi();
+    const int64_t vg = 4 * s->ng;
+    const bool fp64 = s->precision == T2_FP64;
+    if (!s->d_stage)
+        T2_TRY(dalloc_t(s, &s->d_stage, sizeof(double) * n));
+    if (!s->d_src64)
+        T2_TRY(dalloc_t(s, &s->d_src64, sizeof(double) * s->n_phi()));
+    if (fp64 && !s->d_hostang)
+        T2_TRY(dalloc_t(s, &s->d_hostang, sizeof(double) * n));
+    if (!fp64 && !s->d_res)
+        T2_TRY(dalloc_t(s, &s->d_res, sizeof(float) * n));
+    if (psi) {
+        T2_TRY(upload_host(s, s->d_stage, psi, sizeof(double) * n));
+        if (fp64)
+            T2_HIP(t2::launch_ca_to_ac(s->d_stage,
+                                       static_cast<double *>(s->d_psi), s->nc,
+                                       s->na, vg, s->stream));
+        else
+            T2_HIP(t2::launch_ca_to_ac(s->d_stage,
+                                       static_cast<float *>(s->d_psi), s->nc,
+                                       s->na, vg, s->stream));
+    }
+    // q + M S phi in FP64, then the operator apply of the refined solve
+    T2_HIP(t2::launch_total_source<double>(s->d_qiso, s->d_sigs, s->d_phi,
+                                           s->nc, int(s->ng), s->d_src64,
+                                           s->stream));
+    if (fp64) {
+        t2::ResidualArgs<double> ra{};
+        ra.geo = static_cast<const t2::ItemGeo<double> *>(s->d_item_geo);
+        ra.topo = s->d_item_topo;
+        ra.n_items = s->nc * s->na;
+        ra.nc = s->nc;
+        ra.sigma_mode = s->sigma_mode;
+        ra.sigma = static_cast<const double *>(s->d_sigt);
+        ra.src_iso = s->d_src64;
+        ra.src_ang = s->d_qang;
+        ra.lag_up = s->d_lag_up;
+        ra.psi = static_cast<const double *>(s->d_psi);
+        ra.r = s->d_hostang;
+        T2_HIP(t2::launch_residual<double>(ra, int(s->ng), s->stream));
+        T2_HIP(t2::launch_ac_to_ca<double>(s->d_hostang, s->d_stage, s->nc,
+                                           s->na, vg, s->stream));
+    } else {
+        t2::ResidualArgs<float> ra{};
+        ra.geo = s->d_item_geo64;
+        ra.topo = s->d_item_topo;
+        ra.n_items = s->nc * s->na;
+        ra.nc = s->nc;
+        ra.sigma_mode = s->sigma_mode;
+        ra.sigma = s->d_sigt64;
+        ra.src_iso = s->d_src64;
+        ra.src_ang = s->d_qang;
+        ra.lag_up = s->d_lag_up;
+        ra.psi = static_cast<const float *>(s->d_psi);
+        ra.r = s->d_res;
+        T2_HIP(t2::launch_residual<float>(ra, int(s->ng), s->stream));
+        T2_HIP(t2::launch_ac_to_ca<float>(s->d_res, s->d_stage, s->nc, s->na,
+                                          vg, s->stream));
+    }
+    return download_host(s, density, s->d_stage, sizeof(double) * n);
+}
+
 int t2_get_psi_angle(const t2_sweeper *s, int64_t angle, double *psi_a)
 {
     t2::clear_error();

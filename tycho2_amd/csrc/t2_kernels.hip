@@ -778,7 +778,7 @@ residual_kernel(const ResidualArgs<T> p)
         q[v] = p.src_iso[own + v * G + g];
         if (p.src_ang)
             q[v] += p.src_ang[size_t(t.angle) * plane + own + v * G + g];
-        x[v] = double(psi_a[own + v * G + g]);
+        x[v] = double(psi_a[own + v * G + g]);   // FP32 psi: exact widening
     }
     const double sumq = (q[0] + q[1]) + (q[2] + q[3]);
     double r[4];
@@ -823,10 +823,10 @@ residual_kernel(const ResidualArgs<T> p)
     // densities q with weak form V/20 (I + J) q, so store the density
     // (20 / V)(I - J / 5) r that reproduces r exactly.
     const double rs = ((r[0] + r[1]) + (r[2] + r[3])) * 0.2;
-    float *dst = p.r + size_t(t.angle) * plane + own + g;
+    T *dst = p.r + size_t(t.angle) * plane + own + g;
 #pragma unroll
     for (int v = 0; v < 4; v++)
-        dst[v * G] = float((r[v] - rs) / ge_v20);
+        dst[v * G] = T((r[v] - rs) / ge_v20);
 }
 
 // ---------------------------------------------------------------- lagging
@@ -909,8 +909,9 @@ __global__ void total_source_kernel(const double *__restrict__ q,
     }
 }
 
+template <typename T>
 __global__ void ca_to_ac_kernel(const double *__restrict__ in,
-                                double *__restrict__ out, int64_t nc,
+                                T *__restrict__ out, int64_t nc,
                                 int64_t na, int64_t vg)
 {
     const int64_t n = nc * na * vg;
@@ -919,7 +920,7 @@ __global__ void ca_to_ac_kernel(const double *__restrict__ in,
         const int64_t e = i % vg;
         const int64_t ca = i / vg;   // output index (a, c)
         const int64_t c = ca % nc, a = ca / nc;
-        out[i] = in[(c * na + a) * vg + e];
+        out[i] = T(in[(c * na + a) * vg + e]);
     }
 }
 
@@ -1325,10 +1326,11 @@ hipError_t launch_total_source(const double *q, const double *sigma_s,
     return hipGetLastError();
 }
 
-hipError_t launch_ca_to_ac(const double *in, double *out, int64_t nc,
+template <typename T>
+hipError_t launch_ca_to_ac(const double *in, T *out, int64_t nc,
                            int64_t na, int64_t vg, hipStream_t stream)
 {
-    hipLaunchKernelGGL(ca_to_ac_kernel, dim3(stream_grid(nc * na * vg)),
+    hipLaunchKernelGGL((ca_to_ac_kernel<T>), dim3(stream_grid(nc * na * vg)),
                        dim3(256), 0, stream, in, out, nc, na, vg);
     return hipGetLastError();
 }
@@ -1434,6 +1436,8 @@ hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t stream)
                                                const double *, int64_t, int,  \
                                                T *, hipStream_t);             \
     template hipError_t launch_ac_to_ca<T>(const T *, double *, int64_t,      \
+                                           int64_t, int64_t, hipStream_t);    \
+    template hipError_t launch_ca_to_ac<T>(const double *, T *, int64_t,      \
                                            int64_t, int64_t, hipStream_t);
 
 T2_INSTANTIATE(double)

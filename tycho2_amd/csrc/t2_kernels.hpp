@@ -88,8 +88,8 @@ struct SweepArgs {
 // Defect of the coupled transport equations for a stored angular flux,
 // r = q + S phi - L psi, evaluated in FP64 from FP64 item records (lagged
 // faces read the upwind cell's current psi: the operator is the full one),
-// stored as the nodal source density whose weak form is r, so that a sweep
-// with it as source gives L^-1 r.
+// stored (rounded to the flux's type T) as the nodal source density whose
+// weak form is r, so that a sweep with it as source gives L^-1 r.
 template <typename T>
 struct ResidualArgs {
     const ItemGeo<double> *geo;
@@ -102,7 +102,7 @@ struct ResidualArgs {
     const double *src_ang;     // [na][nc][4][G] FP64 or null
     const int32_t *lag_up;     // upwind cell of lagged face i
     const T *psi;
-    float *r;                  // [na][nc][4][G]
+    T *r;                      // [na][nc][4][G]
 };
 
 template <typename T>
@@ -217,7 +217,8 @@ hipError_t launch_total_source(const double *q, const double *sigma_s,
 
 // Layout changes between the reference's [c][a][v][g] and the engine's
 // [a][c][v][g] (vg = 4 * groups contiguous values per (cell, angle)).
-hipError_t launch_ca_to_ac(const double *in, double *out, int64_t nc,
+template <typename T>
+hipError_t launch_ca_to_ac(const double *in, T *out, int64_t nc,
                            int64_t na, int64_t vg, hipStream_t stream);
 template <typename T>
 hipError_t launch_ac_to_ca(const T *in, double *out, int64_t nc, int64_t na,

@@ -71,6 +71,7 @@ SIGNATURES = {
     "t2_solve_refined": (ctypes.c_int, [
         ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
         ctypes.c_int64, ctypes.c_void_p, _dp]),
+    "t2_defect": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "t2_sweep_host": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp,
                                      ctypes.c_int]),
     "t2_set_source_isotropic": (ctypes.c_int, [ctypes.c_void_p, _dp]),
@@ -389,6 +390,18 @@ class SweeperHIP:
             float(rtol_correction), int(restart), ctypes.byref(info),
             _ptr(phi1) if want_phi1 else None))
         return {k: getattr(info, k) for k, _ in T2RefineInfo._fields_}, phi1
+
+    def defect(self, psi: Optional[np.ndarray] = None) -> np.ndarray:
+        """Nodal density d[c][a][v][g] of r = q + M S phi - L psi in FP64
+        (weak form V/20 (I + J) d = r), for the given psi (which becomes the
+        handle's angular flux) or, with None, the handle's own."""
+        out = np.empty((self.nc, self.na, 4, self.ng))
+        p = None
+        if psi is not None:
+            p = _f64(psi).reshape(self.nc, self.na, 4, self.ng)
+        _check(lib().t2_defect(self.handle, _ptr(p) if p is not None
+                               else None, _ptr(out)))
+        return out
 
     def sweep_step(self) -> None:
         _check(lib().t2_sweep_step(self.handle))
