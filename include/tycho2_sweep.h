@@ -252,6 +252,20 @@ int t2_solve_source_iteration(t2_sweeper *s, int64_t iter_max, double err_max,
 int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,
                     int64_t restart, int64_t *n_iter, double *res_hist);
 
+/* The two operator applies of the reference's Krylov path, for callers that
+ * keep their own Krylov solver (PETSc's KSP in src/KrylovSolver.cc:79-169)
+ * and hand it only phi-sized vectors, host [c][v][g] (src/PsiData.hh:369):
+ *   t2_krylov_apply: b = (I - D L^-1 M S) x, lhsOperator of
+ *     src/SourceIteration.cc:76-114 (one sweep with source M S x alone;
+ *     lagged faces read the handle's psi from before the sweep);
+ *   t2_krylov_rhs:   b = D L^-1 q, the right-hand side of
+ *     src/SourceIteration.cc:237-244, with the handle's source.
+ * Both leave the swept psi on the device and the handle's phi unchanged.
+ * After the solve, t2_set_phi(x) and t2_sweep_step give
+ * psi = L^-1 (M S x + q) (src/SourceIteration.cc:254-260). */
+int t2_krylov_apply(t2_sweeper *s, const double *x, double *b);
+int t2_krylov_rhs(t2_sweeper *s, double *b);
+
 /* Mixed-precision solve (handles created with T2_FP32): FP32 sweeps with an
  * FP32 angular flux inside FP64 defect correction.
  *   1. GMRES(restart) in FP64 on phi with FP32 sweeps, to rtol_sweep;

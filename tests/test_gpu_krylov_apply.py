@@ -1,0 +1,95 @@
+"""The Krylov operator applies (t2_krylov_apply, t2_krylov_rhs) against the
+oracle's sweep and moment, and an outside Krylov solver driving the engine
+through them (as the reference's KrylovSolver drives lhsOperator,
+src/SourceIteration.cc:76-114, 216-285), converging to the oracle's GMRES
+answer.  All calls go through the C ABI.
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as sla
+
+import oracle as orc
+from tycho2_amd import mesh as meshmod
+from tycho2_amd.quadrature import product_quadrature
+from tycho2_amd.sweeper import FP32, FP64, SweeperHIP
+
+pytestmark = pytest.mark.gpu
+
+FOUR_PI = 4.0 * np.pi
+
+
+def maxrel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def setup(ng, seed, n=3, sn=4):
+    sm = meshmod.kuhn_cube(n, 0.15, seed=seed)
+    conn = meshmod.tycho_connectivity(sm)
+    omega, wt = product_quadrature(sn)
+    rng = np.random.default_rng(seed)
+    st = rng.uniform(0.5, 3.0, (sm.n_cells, ng))
+    ss = st * rng.uniform(0.1, 0.9, (sm.n_cells, ng))
+    return sm, conn, omega, wt, st, ss, rng
+
+
+def oracle_d_linv(sm, omega, wt, ng, st, src):
+    psi, lag = orc.sweep(orc.build_mesh(sm), omega, ng, st, src)
+    assert lag == 0
+    return orc.psi_to_phi(psi, wt)
+
+
+@pytest.mark.parametrize("precision,ng,bar", [(FP64, 1, 1e-12),
+                                              (FP64, 16, 1e-12),
+                                              (FP32, 4, 2e-5)])
+def test_apply_and_rhs_match_oracle(precision, ng, bar):
+    print(f"krylov apply precision={precision} ng={ng}", flush=True)
+    sm, conn, omega, wt, st, ss, rng = setup(ng, 40 + ng)
+    nc, na = sm.n_cells, omega.shape[0]
+    x = rng.uniform(-0.5, 2.0, (nc, 4, ng))
+    q = rng.uniform(0.0, 2.0, (nc, 4, ng))
+    phi0 = rng.uniform(0.0, 1.0, (nc, 4, ng))
+    with SweeperHIP(sm, conn, omega, wt, ng, st, ss, precision) as s:
+        assert s.info()["n_lagged"] == 0
+        s.set_source_isotropic(q)
+        s.set_phi(phi0)
+        b = s.krylov_apply(x)
+        rhs = s.krylov_rhs()
+        assert np.array_equal(s.phi(), phi0)      # phi is left alone
+    ms_x = (ss[:, None, :] * x)[:, None] / FOUR_PI
+    want_b = x - oracle_d_linv(sm, omega, wt, ng, st,
+                               np.broadcast_to(ms_x, (nc, na, 4, ng)))
+    want_rhs = oracle_d_linv(sm, omega, wt, ng, st,
+                             np.broadcast_to(q[:, None], (nc, na, 4, ng)))
+    assert maxrel(b, want_b) <= bar
+    assert maxrel(rhs, want_rhs) <= bar
+
+
+def test_outside_gmres_drives_the_engine():
+    """scipy's GMRES with the engine as its operator (only phi-sized vectors
+    cross the host link) reaches the oracle's converged phi, and the final
+    sweep gives the oracle's psi, as the reference's krylov() does."""
+    print("outside gmres", flush=True)
+    ng = 4
+    sm, conn, omega, wt, st, ss, rng = setup(ng, 61, n=4)
+    nc, na = sm.n_cells, omega.shape[0]
+    q = rng.uniform(0.5, 2.0, (nc, 4, ng))
+    n = nc * 4 * ng
+    src = np.ascontiguousarray(np.broadcast_to(q[:, None], (nc, na, 4, ng)))
+    rpsi, rphi, _ = orc.krylov(orc.build_mesh(sm), omega, wt, ng, st, ss,
+                               src, 400, 1e-14, 30)
+    with SweeperHIP(sm, conn, omega, wt, ng, st, ss, FP64) as s:
+        assert s.info()["n_lagged"] == 0
+        s.set_source_isotropic(q)
+        op = sla.LinearOperator(
+            (n, n), dtype=np.float64,
+            matvec=lambda v: s.krylov_apply(v.reshape(nc, 4, ng)).ravel())
+        b = s.krylov_rhs().ravel()
+        x, info = sla.gmres(op, b, rtol=1e-13, atol=0.0, restart=30,
+                            maxiter=20)
+        assert info == 0
+        phi = x.reshape(nc, 4, ng)
+        s.set_phi(phi)
+        s.sweep_step()                      # psi = L^-1 (M S phi + q)
+        psi = s.psi()
+    assert maxrel(phi, rphi) <= 1e-11
+    assert maxrel(psi, rpsi) <= 1e-11

@@ -135,6 +135,8 @@ def test_null_handle_errors():
     assert lib.t2_batch_solve_refined(None, 10, 1e-6, 1e-5, 10, None,
                                       None) == -1
     assert lib.t2_defect(None, None, None) == -1
+    assert lib.t2_krylov_apply(None, None, None) == -1
+    assert lib.t2_krylov_rhs(None, None) == -1
     assert lib.t2_batch_destroy(None) == 0
 
 

@@ -86,6 +86,7 @@ struct t2_sweeper {
     double *d_stage = nullptr;       // FP64 psi-sized staging (host path)
     double *d_hostang = nullptr;     // FP64 [na][nc][4][G] (host path)
     double *d_phi = nullptr, *d_phi_old = nullptr;
+    double *d_kvec = nullptr;        // 2 phi-sized vectors (Krylov apply)
     double *d_red = nullptr, *d_scal = nullptr;
     double *d_kry = nullptr;         // Krylov basis + work vectors
     int64_t kry_cols = 0;
@@ -1665,6 +1666,52 @@ int t2_defect(t2_sweeper *s, const double *psi, double *density)
                                           vg, s->stream));
     }
     return download_host(s, density, s->d_stage, sizeof(double) * n);
+}
+
+static int krylov_vectors(t2_sweeper *s)
+{
+    if (!s->d_kvec)
+        T2_TRY(dalloc_t(s, &s->d_kvec, 2 * sizeof(double) * s->n_phi()));
+    return T2_OK;
+}
+
+static int krylov_apply_body(t2_sweeper *s, const double *x, double *b)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (!x || !b)
+        return fail(T2_ERR_ARG, "t2_krylov_apply: null vector");
+    const int64_t n = s->n_phi();
+    T2_TRY(krylov_vectors(s));
+    double *dx = s->d_kvec, *dy = s->d_kvec + n;
+    T2_HIP(hipStreamSynchronize(s->stream));
+    T2_HIP(hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice));
+    T2_TRY(transport_apply(s, dx, dy));
+    return fetch(s, b, dy, n);
+}
+
+static int krylov_rhs_body(t2_sweeper *s, double *b)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (!b)
+        return fail(T2_ERR_ARG, "t2_krylov_rhs: null vector");
+    const int64_t n = s->n_phi();
+    T2_TRY(krylov_vectors(s));
+    T2_TRY(total_source(s, s->d_qiso, nullptr));
+    T2_TRY(run_sweep(s, s->d_qang));
+    T2_TRY(psi_to_phi(s, s->d_kvec));
+    return fetch(s, b, s->d_kvec, n);
+}
+
+int t2_krylov_apply(t2_sweeper *s, const double *x, double *b)
+{
+    return after_queue(s, krylov_apply_body(s, x, b));
+}
+
+int t2_krylov_rhs(t2_sweeper *s, double *b)
+{
+    return after_queue(s, krylov_rhs_body(s, b));
 }
 
 int t2_get_psi_angle(const t2_sweeper *s, int64_t angle, double *psi_a)

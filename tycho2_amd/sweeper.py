@@ -72,6 +72,8 @@ SIGNATURES = {
         ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
         ctypes.c_int64, ctypes.c_void_p, _dp]),
     "t2_defect": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "t2_krylov_apply": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "t2_krylov_rhs": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "t2_sweep_host": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp,
                                      ctypes.c_int]),
     "t2_set_source_isotropic": (ctypes.c_int, [ctypes.c_void_p, _dp]),
@@ -390,6 +392,20 @@ class SweeperHIP:
             float(rtol_correction), int(restart), ctypes.byref(info),
             _ptr(phi1) if want_phi1 else None))
         return {k: getattr(info, k) for k, _ in T2RefineInfo._fields_}, phi1
+
+    def krylov_apply(self, x: np.ndarray) -> np.ndarray:
+        """b = (I - D L^-1 M S) x for phi-shaped x [c][v][g] (the
+        reference's lhsOperator); one sweep on the device."""
+        p = _f64(x).reshape(self.nc, 4, self.ng)
+        out = np.empty((self.nc, 4, self.ng))
+        _check(lib().t2_krylov_apply(self.handle, _ptr(p), _ptr(out)))
+        return out
+
+    def krylov_rhs(self) -> np.ndarray:
+        """b = D L^-1 q with the handle's source (one sweep)."""
+        out = np.empty((self.nc, 4, self.ng))
+        _check(lib().t2_krylov_rhs(self.handle, _ptr(out)))
+        return out
 
     def defect(self, psi: Optional[np.ndarray] = None) -> np.ndarray:
         """Nodal density d[c][a][v][g] of r = q + M S phi - L psi in FP64
