@@ -81,6 +81,15 @@ def test_driver_reproduces_gold(tmp_path, gold, extra, gauss):
     l1, linf = psi_io.rel_diff(got, gold["psi_g0"])
     assert l1 < 1e-10 and linf < 1e-10
     assert "iterations: 34" in r.stdout
+    # the reference executable's end-of-run lines (src/Main.cc:340-345)
+    lines = dict(line.split(": ", 1) for line in r.stdout.splitlines()
+                 if line.startswith(("L2 Relative Error", "Diff between")))
+    from tycho2_amd import mesh as meshmod, problems
+    sm = meshmod.read_smesh(golden("cube-208.smesh"))
+    assert float(lines["L2 Relative Error"]) == pytest.approx(
+        problems.hat_l2_error(sm, psi), rel=1e-6)
+    assert float(lines["Diff between groups"]) == \
+        psi_io.diff_between_groups(psi) == 0.0
 
 
 @pytest.mark.gpu

@@ -157,3 +157,40 @@ def test_pmesh_bytes_equal_reference_writer(tmp_path, kind, name):
     got = tmp_path / "ours.pmesh"
     meshmod.write_pmesh(sm, str(got), part, n)
     assert got.read_bytes() == want.read_bytes()
+
+
+def test_diff_between_groups_closed_form():
+    """0 for identical groups and for one group; one perturbed entry of
+    size d gives d / max|psi_0| whatever group holds it."""
+    rng = np.random.default_rng(5)
+    base = rng.uniform(-1.0, 2.0, (7, 3, 4, 1))
+    psi = np.repeat(base, 4, axis=3)
+    assert psi_io.diff_between_groups(psi) == 0.0
+    assert psi_io.diff_between_groups(base) == 0.0
+    psi[5, 1, 2, 3] += 0.25
+    psi[2, 0, 1, 1] -= 0.125
+    want = 0.25 / np.abs(base).max()
+    assert psi_io.diff_between_groups(psi) == pytest.approx(want, rel=1e-15)
+
+
+def test_hat_l2_error_closed_form(cube208):
+    """psi equal to the hat solution at each cell's centroid gives 0 (to
+    round-off); psi scaled by 1 + e gives e / (1 + e); cells outside r = 30
+    count in neither sum."""
+    from tycho2_amd import problems
+    sm = cube208
+    cen = sm.coords[sm.cell_nodes].mean(axis=1) - 50.0
+    r = np.sqrt((cen * cen).sum(axis=1))
+    assert (r > 30.0).any() and (r <= 30.0).any()
+    hat = np.where(r <= 30.0, 1.0 - r / 30.0, 0.0)
+    psi = np.broadcast_to(hat[:, None, None, None],
+                          (sm.n_cells, 8, 4, 2)).copy()
+    assert problems.hat_l2_error(sm, psi) <= 1e-15
+    e = 0.01
+    assert problems.hat_l2_error(sm, (1.0 + e) * psi) == pytest.approx(
+        e / (1.0 + e), rel=1e-12)
+    # a vertex-varying psi with the hat as its cell average
+    tilt = np.array([0.3, -0.1, -0.15, -0.05])
+    psi2 = psi + tilt[None, None, :, None] * hat[:, None, None, None]
+    assert problems.hat_l2_error(sm, psi2) <= 1e-15
+

@@ -15,6 +15,7 @@
 //
 // Usage: sweep_hip.x <mesh.smesh | mesh.pmesh> <input.deck>
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -515,6 +516,51 @@ void SweeperHIP::solve()
     check(t2_get_psi(c_engine, psi.v.data()));
 }
 
+// End-of-run checks the reference's executable prints (src/Main.cc:340-345).
+// L2 relative error of the cell-average psi against the hat solution at the
+// cell centroid of [0,100]^3 (src/Problem.cc:91-134).
+double hat_l2_error(const Problem &p, PsiData &psi)
+{
+    double diff = 0.0, norm = 0.0;
+    for (int64_t c = 0; c < p.mesh.nc; c++)
+        for (int64_t a = 0; a < p.na; a++)
+            for (int64_t g = 0; g < p.ng; g++) {
+                double x = 0.0, y = 0.0, z = 0.0, val = 0.0;
+                for (int v = 0; v < 4; v++) {
+                    const double *n = &p.mesh.coords[3 * p.mesh.cell_nodes[4 * c + v]];
+                    x += n[0] - 50.0;
+                    y += n[1] - 50.0;
+                    z += n[2] - 50.0;
+                    val += psi(g, v, a, c);
+                }
+                x /= 4.0;
+                y /= 4.0;
+                z /= 4.0;
+                val /= 4.0;
+                const double r = std::sqrt(x * x + y * y + z * z);
+                const double act = r <= 30.0 ? 1.0 - r / 30.0 : 0.0;
+                norm += val * val;
+                diff += (val - act) * (val - act);
+            }
+    return std::sqrt(diff / norm);
+}
+
+// Largest |psi_g - psi_0| (g > 0) relative to the largest |psi_0|
+// (src/Util.cc:55-79).
+double diff_between_groups(PsiData &psi)
+{
+    double max_diff = 0.0, max_entry = 0.0;
+    for (int64_t c = 0; c < psi.nc; c++)
+        for (int64_t a = 0; a < psi.na; a++)
+            for (int v = 0; v < 4; v++) {
+                const double p0 = psi(0, v, a, c);
+                max_entry = std::max(max_entry, std::fabs(p0));
+                for (int64_t g = 1; g < psi.ng; g++)
+                    max_diff = std::max(max_diff, std::fabs(p0 - psi(g, v, a, c)));
+            }
+    return max_diff / max_entry;
+}
+
 // "Tycho 2 Psi Output" v1 (src/PsiData.cc:46-106)
 void write_psi(const std::string &path, const PsiData &psi)
 {
@@ -565,8 +611,15 @@ int main(int argc, char **argv)
         }
         SweeperHIP sweeper(p, deck);
         SweeperAbstract &abstract = sweeper;
+        const auto t0 = std::chrono::steady_clock::now();
         abstract.solve();
+        const std::chrono::duration<double> dt =
+            std::chrono::steady_clock::now() - t0;
         std::printf("iterations: %lld\n", (long long)sweeper.iterations);
+        std::printf("Total time: %.2f\n", dt.count());
+        std::printf("L2 Relative Error: %e\n", hat_l2_error(p, sweeper.psi));
+        std::printf("Diff between groups: %e\n",
+                    diff_between_groups(sweeper.psi));
         if (deck.flag("OutputFile"))
             write_psi(deck.str("OutputFilename"), sweeper.psi);
     } catch (const std::exception &e) {

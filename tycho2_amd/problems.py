@@ -154,3 +154,18 @@ def hat_source(p: Problem) -> np.ndarray:
     val = np.where((r <= 30.0)[:, None, :], stream + absorb[:, None, :], 0.0)
     return np.ascontiguousarray(
         np.broadcast_to(val[..., None], (p.nc, p.na, 4, p.ng)))
+
+
+def hat_l2_error(smesh: meshmod.SerialMesh, psi: np.ndarray) -> float:
+    """L2 relative error of the cell-average psi against the hat solution
+    1 - r/30 (r <= 30, else 0) at the cell centroid of [0,100]^3, over all
+    (cell, angle, group): the reference's end-of-run check
+    (src/Problem.cc:91-134, printed by src/Main.cc:340-343)."""
+    psi = np.asarray(psi, dtype=np.float64)
+    xyz = (smesh.coords[smesh.cell_nodes] - 50.0).sum(axis=1) / 4.0
+    c = np.sqrt((xyz * xyz).sum(axis=1))                       # (nc,)
+    act = np.where(c <= 30.0, 1.0 - c / 30.0, 0.0)
+    val = psi.sum(axis=2) / 4.0                                # (nc, na, ng)
+    diff = val - act[:, None, None]
+    return float(np.sqrt((diff * diff).sum() / (val * val).sum()))
+

@@ -45,3 +45,15 @@ def rel_diff(a: np.ndarray, b: np.ndarray):
     d = a - b
     return (float(np.abs(d).sum() / np.abs(b).sum()),
             float(np.abs(d).max() / np.abs(b).max()))
+
+
+def diff_between_groups(psi: np.ndarray) -> float:
+    """max |psi_g - psi_0| over groups g > 0, relative to max |psi_0|, for
+    psi[cell][angle][vertex][group]: the reference's end-of-run check
+    (src/Util.cc:55-79, printed by src/Main.cc:341-344)."""
+    psi = np.asarray(psi, dtype=np.float64)
+    p0 = psi[..., 0]
+    d = np.abs(psi[..., 1:] - p0[..., None]).max() if psi.shape[-1] > 1 \
+        else 0.0
+    return float(d / np.abs(p0).max())
+
