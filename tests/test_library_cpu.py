@@ -148,3 +148,8 @@ def test_geometry_rejects_bad_face_map():
     with pytest.raises(sweeper.T2Error):
         sweeper.mesh_geometry(sm, meshmod.Connectivity(con.adj_cell, bad,
                                                        con.nbr_vrtx))
+    dup = con.f2c.copy()
+    dup[2, 1, 2] = dup[2, 1, 0]   # a face vertex twice: maps not inverse
+    with pytest.raises(sweeper.T2Error, match="vertex map"):
+        sweeper.mesh_geometry(sm, meshmod.Connectivity(con.adj_cell, dup,
+                                                       con.nbr_vrtx))

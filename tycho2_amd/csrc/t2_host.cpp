@@ -62,10 +62,15 @@ int mesh_geometry(int64_t nc, int64_t nn, const double *coords,
         volume[c] = 1.0 / 6.0 * std::fabs(x.x * e3.x + x.y * e3.y + x.z * e3.z);
         for (int k = 0; k < 4; k++) {
             V3 f[3];
+            // face k holds the three cell vertices other than k, each once,
+            // so that the face and cell vertex maps are inverse
+            // (src/TychoMesh.cc:227-241)
+            unsigned seen = 0;
             for (int j = 0; j < 3; j++) {
                 int64_t cv = f2c[(4 * c + k) * 3 + j];
-                if (cv < 0 || cv > 3 || cv == k)
+                if (cv < 0 || cv > 3 || cv == k || (seen >> cv & 1u))
                     return fail(T2_ERR_ARG, "bad face-to-cell vertex map");
+                seen |= 1u << cv;
                 f[j] = p[cv];
             }
             V3 n = cross(f[1] - f[0], f[2] - f[0]);
