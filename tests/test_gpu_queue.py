@@ -49,6 +49,25 @@ def run_steps(args, precision, engine, steps=3, batches=1, launch=None,
         return s.phi(), s.psi_angle(omega.shape[0] - 1)
 
 
+@pytest.mark.parametrize("engine", [ENGINE_LEVELS, ENGINE_QUEUE,
+                                    ENGINE_HYBRID])
+@pytest.mark.parametrize("precision", [FP64, FP32])
+@pytest.mark.parametrize("lagged", [False, True], ids=["acyclic", "lagged"])
+def test_groups_per_lane_bitwise(engine, precision, lagged):
+    """Two and four groups per lane (all groups' loads issued before any
+    group is solved and stored) give one group per lane's results bit for
+    bit on every engine, with and without lagged faces."""
+    print(f"groups per lane engine={engine} precision={precision} "
+          f"lagged={lagged}", flush=True)
+    args = (problem(3, 2, 8, seed=3, jitter=0.2) if lagged
+            else problem(4, 4, 8, seed=44))
+    ref = run_steps(args, precision, engine, launch=(8, 1))
+    for launch in [(16, 2), (32, 4)]:
+        got = run_steps(args, precision, engine, launch=launch)
+        assert np.array_equal(ref[0], got[0])
+        assert np.array_equal(ref[1], got[1])
+
+
 @pytest.mark.parametrize("precision,ng", [(FP64, 1), (FP64, 4), (FP64, 16),
                                           (FP32, 1), (FP32, 16)])
 def test_queue_matches_levels_bitwise(precision, ng):
