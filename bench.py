@@ -533,6 +533,22 @@ def maxrel(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
+ELEM_FLOOR = 1e-6   # of max |phi_ref|: the floor of the elementwise error
+
+
+def phi_errors(a, b):
+    """The other two error measures of phi against the reference: L1
+    relative, sum |a - b| / sum |b|, and elementwise relative,
+    max |a - b| / max(|b|, ELEM_FLOOR max |b|)."""
+    import numpy as np
+    d = np.abs(a - b)
+    ab = np.abs(b)
+    floor = ELEM_FLOOR * ab.max()
+    return {"l1_rel": float(d.sum() / ab.sum()),
+            "elem_maxrel": float((d / np.maximum(ab, floor)).max()),
+            "elem_floor": ELEM_FLOOR}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -684,6 +700,7 @@ def main() -> int:
         e1 = maxrel(r32["phi1"], ref)
         e2 = maxrel(r32["phi"], ref)
         plain["phi_maxrel_err"] = maxrel(r64["phi"], ref)
+        plain["phi_err_other"] = phi_errors(r64["phi"], ref)
         plain["error_reference"] = ("FP64 GMRES(20) to 1e-13 on the same "
                                     "handle (untimed)")
         refined.update(solve_s=round(r32["solve_ms"] / 1e3, 3),
@@ -692,6 +709,10 @@ def main() -> int:
                                                 (r32["solve_ms"] * 1e-3)),
                        phi_maxrel_err_after_fp32_stage=e1,
                        phi_maxrel_err_after_correction=e2,
+                       phi_err_other_after_fp32_stage=phi_errors(
+                           r32["phi1"], ref),
+                       phi_err_other_after_correction=phi_errors(
+                           r32["phi"], ref),
                        gmres_iters=[r32["refine"]["iters_sweep"],
                                     r32["refine"]["iters_correction"]])
         refined["speedup_solves_per_s"] = round(

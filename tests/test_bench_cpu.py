@@ -73,3 +73,26 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode == 2
     assert "one GPU" in r.stderr
     assert r.stdout == ""
+
+
+def test_phi_error_measures():
+    """A uniform relative perturbation e gives e in every measure; an
+    entry of the reference below the floor is measured against the floor,
+    and the L1 measure weighs entries by size."""
+    rng = np.random.default_rng(2)
+    ref = rng.uniform(0.5, 2.0, (30, 4, 3))
+    e = 1e-9
+    got = bench.phi_errors(ref * (1 + e), ref)
+    assert got["l1_rel"] == pytest.approx(e, rel=1e-6)
+    assert got["elem_maxrel"] == pytest.approx(e, rel=1e-6)
+    ref2 = ref.copy()
+    ref2[3, 1, 2] = 0.0
+    a = ref2.copy()
+    a[3, 1, 2] = 1e-9
+    floor = bench.ELEM_FLOOR * np.abs(ref2).max()
+    got = bench.phi_errors(a, ref2)
+    assert got["elem_maxrel"] == pytest.approx(1e-9 / floor, rel=1e-12)
+    assert got["l1_rel"] == pytest.approx(1e-9 / np.abs(ref2).sum(),
+                                          rel=1e-12)
+    assert bench.maxrel(a, ref2) == pytest.approx(
+        1e-9 / np.abs(ref2).max(), rel=1e-12)
