@@ -574,6 +574,14 @@ int zero_psi(t2_sweeper *s)
     return T2_OK;
 }
 
+// psi = 0 at the start of a solve whose first step is a full sweep: before
+// that sweep has written all of psi only the lagged faces read it (their
+// pre-sweep snapshot), so without lagged faces the fill is skipped.
+int zero_psi_before_sweep(t2_sweeper *s)
+{
+    return s->sched.lagged.empty() ? T2_OK : zero_psi(s);
+}
+
 // ---------------------------------------------------------- host link
 // psi-sized transfers between the caller's ordinary (pageable) arrays and
 // the device go through two pinned chunks: host threads copy one chunk
@@ -1464,7 +1472,7 @@ static int solve_krylov_body(t2_sweeper *s, int64_t iter_max, double rtol,
     double *V = nullptr, *work = nullptr;
     T2_TRY(krylov_space(s, restart, 1, &V, &work));
     double *b = work;
-    T2_TRY(zero_psi(s));
+    T2_TRY(zero_psi_before_sweep(s));
     // b = D L^-1 q
     T2_TRY(total_source(s, s->d_qiso, nullptr));
     T2_TRY(run_sweep(s, s->d_qang));
@@ -1498,7 +1506,7 @@ static int solve_refined_body(t2_sweeper *s, int64_t iter_max, double rtol_sweep
         T2_TRY(dalloc_t(s, &s->d_src64, sizeof(double) * n));
     t2_refine_info st{};
     const int64_t sweeps0 = s->n_sweeps;
-    T2_TRY(zero_psi(s));
+    T2_TRY(zero_psi_before_sweep(s));
 
     // 1. FP32-sweep GMRES on phi, then psi32 and phi1 = D psi32
     T2_TRY(total_source(s, s->d_qiso, nullptr));
@@ -2649,7 +2657,7 @@ int t2_batch_solve_krylov(t2_batch *b, int64_t iter_max, double rtol,
         double *work = nullptr;
         T2_TRY(krylov_space(m, restart, 1, &V[i], &work));
         rhs[i] = work;
-        T2_TRY(zero_psi(m));
+        T2_TRY(zero_psi_before_sweep(m));
         T2_TRY(total_source(m, m->d_qiso, nullptr));  // b = D L^-1 q
     }
     std::vector<int32_t> act(nm, 1);
@@ -2702,7 +2710,7 @@ static int batch_refined_body(t2_batch *b, int64_t iter_max,
         if (!m->d_src64)
             T2_TRY(dalloc_t(m, &m->d_src64, sizeof(double) * n));
         sweeps0[i] = m->n_sweeps;
-        T2_TRY(zero_psi(m));
+        T2_TRY(zero_psi_before_sweep(m));
         T2_TRY(total_source(m, m->d_qiso, nullptr));
     }
     std::vector<int32_t> all(nm, 1);
