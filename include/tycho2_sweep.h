@@ -265,6 +265,13 @@ int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,
  * psi = L^-1 (M S x + q) (src/SourceIteration.cc:254-260). */
 int t2_krylov_apply(t2_sweeper *s, const double *x, double *b);
 int t2_krylov_rhs(t2_sweeper *s, double *b);
+/* t2_krylov_apply on device vectors (memory of the handle's device, n_cells
+ * x 4 x n_groups doubles each, not overlapping), for a Krylov solver that
+ * keeps its vectors on the GPU: the work is enqueued on the handle's stream
+ * (t2_get_stream) after everything already there, and the call returns
+ * without waiting.  The caller orders its own writes of x before the call
+ * and its reads of b after it (e.g. t2_synchronize). */
+int t2_krylov_apply_device(t2_sweeper *s, const double *x, double *b);
 
 /* Mixed-precision solve (handles created with T2_FP32): FP32 sweeps with an
  * FP32 angular flux inside FP64 defect correction.

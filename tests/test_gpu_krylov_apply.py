@@ -132,3 +132,42 @@ def test_solve_ignores_the_previous_psi(precision, lagged):
         second = solve()
     for a, b in zip(first, second):
         assert np.array_equal(a, b)
+
+
+def test_device_apply_equals_host_apply():
+    """The apply on device vectors (allocated with the HIP runtime the
+    library uses) gives the host apply's b bit for bit; aliasing or
+    overlapping vectors are refused."""
+    import ctypes
+    from tycho2_amd.sweeper import T2Error
+    print("krylov apply on device vectors", flush=True)
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
+                              ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    h2d, d2h = 1, 2
+    ng = 4
+    sm, conn, omega, wt, st, ss, rng = setup(ng, 77)
+    x = rng.uniform(-0.5, 2.0, (sm.n_cells, 4, ng))
+    got = np.zeros_like(x)
+    dx, db = ctypes.c_void_p(), ctypes.c_void_p()
+    with SweeperHIP(sm, conn, omega, wt, ng, st, ss, FP64) as s:
+        assert s.info()["n_lagged"] == 0
+        want = s.krylov_apply(x)
+        assert hip.hipMalloc(ctypes.byref(dx), x.nbytes) == 0
+        assert hip.hipMalloc(ctypes.byref(db), x.nbytes) == 0
+        try:
+            assert hip.hipMemcpy(dx, x.ctypes.data, x.nbytes, h2d) == 0
+            s.synchronize()
+            s.krylov_apply_device(dx.value, db.value)
+            s.synchronize()
+            assert hip.hipMemcpy(got.ctypes.data, db, x.nbytes, d2h) == 0
+            with pytest.raises(T2Error):
+                s.krylov_apply_device(dx.value, dx.value)
+            with pytest.raises(T2Error):
+                s.krylov_apply_device(dx.value, dx.value + 8)
+        finally:
+            hip.hipFree(dx)
+            hip.hipFree(db)
+    assert np.array_equal(got, want)

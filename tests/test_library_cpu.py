@@ -137,6 +137,7 @@ def test_null_handle_errors():
     assert lib.t2_defect(None, None, None) == -1
     assert lib.t2_krylov_apply(None, None, None) == -1
     assert lib.t2_krylov_rhs(None, None) == -1
+    assert lib.t2_krylov_apply_device(None, None, None) == -1
     assert lib.t2_batch_destroy(None) == 0
 
 

@@ -1717,6 +1717,20 @@ int t2_krylov_apply(t2_sweeper *s, const double *x, double *b)
     return after_queue(s, krylov_apply_body(s, x, b));
 }
 
+int t2_krylov_apply_device(t2_sweeper *s, const double *x, double *b)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (!x || !b)
+        return fail(T2_ERR_ARG, "t2_krylov_apply_device: null vector");
+    if (x == b)
+        return fail(T2_ERR_ARG, "t2_krylov_apply_device: x and b alias");
+    const int64_t n = s->n_phi();
+    if ((x < b && x + n > b) || (b < x && b + n > x))
+        return fail(T2_ERR_ARG, "t2_krylov_apply_device: x and b overlap");
+    return transport_apply(s, x, b);
+}
+
 int t2_krylov_rhs(t2_sweeper *s, double *b)
 {
     return after_queue(s, krylov_rhs_body(s, b));

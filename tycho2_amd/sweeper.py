@@ -74,6 +74,9 @@ SIGNATURES = {
     "t2_defect": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "t2_krylov_apply": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "t2_krylov_rhs": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "t2_krylov_apply_device": (ctypes.c_int, [ctypes.c_void_p,
+                                              ctypes.c_void_p,
+                                              ctypes.c_void_p]),
     "t2_sweep_host": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp,
                                      ctypes.c_int]),
     "t2_set_source_isotropic": (ctypes.c_int, [ctypes.c_void_p, _dp]),
@@ -400,6 +403,14 @@ class SweeperHIP:
         out = np.empty((self.nc, 4, self.ng))
         _check(lib().t2_krylov_apply(self.handle, _ptr(p), _ptr(out)))
         return out
+
+    def krylov_apply_device(self, x_ptr: int, b_ptr: int) -> None:
+        """b = (I - D L^-1 M S) x on device vectors (addresses of n_cells x 4
+        x n_groups doubles on the handle's device), enqueued on the
+        handle's stream; synchronize before reading b."""
+        _check(lib().t2_krylov_apply_device(self.handle,
+                                            ctypes.c_void_p(x_ptr),
+                                            ctypes.c_void_p(b_ptr)))
 
     def krylov_rhs(self) -> np.ndarray:
         """b = D L^-1 q with the handle's source (one sweep)."""
