@@ -351,6 +351,10 @@ int t2_batch_set_engine(t2_batch *b, int engine, int chunk_rounds,
  * count's members dealt out by work; 0 (default) interleaves every launch
  * on one stream.  Results do not depend on it. */
 int t2_batch_set_concurrent(t2_batch *b, int chains);
+/* Level form: threads per block of the merged step launches (a multiple of
+ * 64 in [64, 256]; default 256), one group per lane.  Results do not depend
+ * on it (autotuner knob). */
+int t2_batch_set_launch(t2_batch *b, int block);
 /* t2_sweep_step / t2_sweep_only of every member in one batched sweep */
 int t2_batch_sweep_step(t2_batch *b);
 int t2_batch_sweep_only(t2_batch *b);

@@ -457,10 +457,39 @@ def test_batch_priority_members_bitwise(engine):
         s.close()
 
 
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_batch_block_sizes_bitwise(precision):
+    """The level form's merged launches give the members' own sweeps bit
+    for bit at 64, 128 and 256 threads per block, with the group counts
+    interleaved and on parallel streams."""
+    from tycho2_amd.sweeper import SweepBatch
+    print(f"batch block sizes precision={precision}", flush=True)
+    groups = (4, 16, 1)
+    own = batch_members(precision, groups)
+    for s in own:
+        s.sweep_step()
+    want = [s.phi() for s in own]
+    for block, chains in [(64, 0), (128, 1), (256, 0)]:
+        mem = batch_members(precision, groups)
+        with SweepBatch(mem) as b:
+            b.set_engine(0, 1 << 20)
+            b.set_concurrent(chains)
+            b.set_launch(block)
+            b.sweep_step()
+            b.synchronize()
+        for w, m in zip(want, mem):
+            assert np.array_equal(w, m.phi())
+        for m in mem:
+            m.close()
+    for s in own:
+        s.close()
+
+
 def test_batch_autotune_keeps_results():
-    """The batch tuner times both engines and the queue's chunk sizes,
-    leaves the fastest on the batch, and the batch's steps afterwards equal
-    the members' own bit for bit."""
+    """The batch tuner times both engines, the level form's chain counts
+    and block sizes and the queue's chunk sizes, leaves the fastest on the
+    batch, and the batch's steps afterwards equal the members' own bit for
+    bit (whatever block size it kept)."""
     from tycho2_amd import autotune
     from tycho2_amd.sweeper import SweepBatch
     print("batch autotune", flush=True)
@@ -474,9 +503,11 @@ def test_batch_autotune_keeps_results():
         assert best == min(trials, key=lambda t: t["ms"])
         assert {t["engine"] for t in trials} == {0, 1}
         assert len(trials) == len(autotune.CHAINS) + len(
-            autotune.CHUNK_ROUNDS)
+            autotune.CHUNK_ROUNDS) + len(autotune.BLOCKS) - 1
         assert {t["concurrent"] for t in trials if t["engine"] == 0} == \
             set(autotune.CHAINS)
+        assert {t["block"] for t in trials if t["engine"] == 0} == \
+            set(autotune.BLOCKS)
         for m in mem:
             m.set_phi(np.zeros_like(m.phi()))
         b.sweep_step()

@@ -132,36 +132,52 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
 
 
 def tune_batch(b, reps: int = 3, rounds=CHUNK_ROUNDS,
-               engines=(ENGINE_LEVELS, ENGINE_QUEUE), log=None):
+               engines=(ENGINE_LEVELS, ENGINE_QUEUE), log=None,
+               blocks=BLOCKS):
     """Tune batch `b` in place over its engine (merged per-step launches,
     their group counts interleaved or on parallel streams, or one
-    persistent ready-queue launch) and the queue's chunk size; returns
-    (best, trials) like tune().  A level form that refuses the members'
-    schedules (too many steps) is skipped."""
+    persistent ready-queue launch), the level form's block size (at its
+    best chain count) and the queue's chunk size; returns (best, trials)
+    like tune().  A level form that refuses the members' schedules (too
+    many steps) is skipped."""
     from .sweeper import T2Error
     ev = HipEvents()
     trials = []
-    for engine in engines:
-        variants = ([(r, 0) for r in rounds] if engine == ENGINE_QUEUE
-                    else [(0, c) for c in CHAINS])
-        for r, conc in variants:
-            b.set_engine(engine, 0, r)
-            b.set_concurrent(conc)
-            try:
-                b.sweep_only()
-            except T2Error as e:
-                if log:
-                    log(f"tune batch engine={engine}: {e}")
-                continue
-            b.synchronize()
-            ms = time_region(ev, b, b.sweep_only, reps) / reps
-            t = dict(engine=engine, chunk_rounds=r, concurrent=conc, ms=ms)
-            trials.append(t)
+
+    def trial(engine, r, conc, block):
+        b.set_engine(engine, 0, r)
+        b.set_concurrent(conc)
+        b.set_launch(block)
+        try:
+            b.sweep_only()
+        except T2Error as e:
             if log:
-                log(f"tune batch {t}")
+                log(f"tune batch engine={engine}: {e}")
+            return
+        b.synchronize()
+        ms = time_region(ev, b, b.sweep_only, reps) / reps
+        t = dict(engine=engine, chunk_rounds=r, concurrent=conc,
+                 block=block, ms=ms)
+        trials.append(t)
+        if log:
+            log(f"tune batch {t}")
+
+    for engine in engines:
+        if engine == ENGINE_QUEUE:
+            for r in rounds:
+                trial(engine, r, 0, blocks[0])
+            continue
+        for conc in CHAINS:
+            trial(engine, 0, conc, blocks[0])
+        level = [t for t in trials if t["engine"] == engine]
+        if level:
+            conc = min(level, key=lambda t: t["ms"])["concurrent"]
+            for block in blocks[1:]:
+                trial(engine, 0, conc, block)
     if not trials:
         raise RuntimeError("no batch engine could sweep these members")
     best = min(trials, key=lambda t: t["ms"])
     b.set_engine(best["engine"], 0, best["chunk_rounds"])
     b.set_concurrent(best["concurrent"])
+    b.set_launch(best["block"])
     return best, trials

@@ -1874,6 +1874,7 @@ struct t2_batch {
     int engine = T2_ENGINE_LEVELS;
     int concurrent = 0;                  // level form: chains per group count
                                          // (0: all launches on one stream)
+    int block = t2::kMaxBlock;           // level form: threads per block
     std::vector<hipStream_t> side;       // their extra streams
     std::vector<hipEvent_t> join_ev;
     hipEvent_t fork_ev = nullptr;
@@ -2247,7 +2248,7 @@ int batch_enqueue(t2_batch *b)
         a.geo = static_cast<const t2::ItemGeo<T> *>(g.d_geo) + g.step_off[st];
         a.topo = g.d_topo + g.step_off[st];
         a.n_items = g.step_off[st + 1] - g.step_off[st];
-        T2_HIP(t2::launch_sweep_step<T>(a, g.ng, 1, t2::kMaxBlock, stream));
+        T2_HIP(t2::launch_sweep_step<T>(a, g.ng, 1, b->block, stream));
         return T2_OK;
     };
     const size_t ng = b->groups.size();
@@ -2500,6 +2501,23 @@ int t2_batch_destroy(t2_batch *b)
     if (b->stream)
         (void)hipStreamDestroy(b->stream);
     delete b;
+    return T2_OK;
+}
+
+int t2_batch_set_launch(t2_batch *b, int block)
+{
+    t2::clear_error();
+    if (!b)
+        return fail(T2_ERR_ARG, "null batch");
+    if (block < 64 || block > t2::kMaxBlock || block % 64)
+        return fail(T2_ERR_ARG, "block must be a multiple of 64 in [64, " +
+                                    std::to_string(t2::kMaxBlock) + "]");
+    if (block == b->block)
+        return T2_OK;
+    T2_HIP(hipSetDevice(b->device));
+    T2_HIP(hipStreamSynchronize(b->stream));
+    drop_batch_graph(b);
+    b->block = block;
     return T2_OK;
 }
 

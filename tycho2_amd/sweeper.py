@@ -104,6 +104,7 @@ SIGNATURES = {
                                            ctypes.c_int, ctypes.c_int64]),
     "t2_batch_set_concurrent": (ctypes.c_int, [ctypes.c_void_p,
                                                ctypes.c_int]),
+    "t2_batch_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "t2_batch_sweep_step": (ctypes.c_int, [ctypes.c_void_p]),
     "t2_batch_sweep_only": (ctypes.c_int, [ctypes.c_void_p]),
     "t2_batch_info": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip]),
@@ -510,6 +511,10 @@ class SweepBatch:
                    chunk_rounds: int = 0) -> None:
         _check(lib().t2_batch_set_engine(self.handle, int(engine),
                                          int(chunk_rounds), int(spin_limit)))
+
+    def set_launch(self, block: int) -> None:
+        """Threads per block of the level form's merged launches."""
+        _check(lib().t2_batch_set_launch(self.handle, int(block)))
 
     def set_concurrent(self, chains: int) -> None:
         """Level form: independent merged schedules on parallel streams,
