@@ -55,8 +55,12 @@ def fixed_launch(s, launch, batches=1, concurrent=False):
     """Apply a "block,groups_per_lane" launch shape (threads per block) and
     an angle-batch schedule (one after the other or concurrent)."""
     if launch:
-        block, gpl = (int(x) for x in launch.split(","))
+        vals = [int(x) for x in launch.split(",")]
+        block, gpl = vals[0], vals[1]
+        s.set_items_per_lane(1)
         s.set_launch(block // (s.ng // gpl), gpl)
+        if len(vals) > 2:
+            s.set_items_per_lane(vals[2])
     if batches > 1:
         s.set_schedule(batches)
     s.set_concurrent_batches(concurrent)
@@ -582,8 +586,9 @@ def main() -> int:
                     help="with --no-tune: sweep the angle batches on "
                          "parallel streams")
     ap.add_argument("--launch", default="",
-                    help="with --no-tune: threads per block and groups per "
-                         "lane of the sweep kernels, e.g. 64,1")
+                    help="with --no-tune: threads per block, groups per "
+                         "lane and optionally items per lane of the sweep "
+                         "kernels, e.g. 64,1 or 64,1,2")
     ap.add_argument("--host-path", action="store_true",
                     help="only time the drop-in host call (PCIe-inclusive)")
     ap.add_argument("--schedule-study", action="store_true",

@@ -163,6 +163,15 @@ int t2_sweeper_info(const t2_sweeper *s, t2_info *info);
  * 0 keeps the default. */
 int t2_set_launch(t2_sweeper *s, int items_per_block, int groups_per_lane);
 
+/* Items per lane of the level engine's step launches (1 or 2; NoPivot cell
+ * solver and one group per lane only): with 2 a wavefront solves two sets
+ * of items, all of both sets' loads issued before either set is solved, so
+ * that twice the loads are in flight at the same occupancy.  Results are
+ * bitwise those of one item per lane (autotuner knob; the ready queue
+ * ignores it).  No reference counterpart: it maps the reference's cell
+ * loop (src/GraphTraverser.cc:467-759) onto wavefront lanes. */
+int t2_set_items_per_lane(t2_sweeper *s, int items);
+
 /* Sweep schedule: the angles are swept in `angle_batches` consecutive
  * batches (octant by octant, then by index), each batch level by level.  One
  * batch runs every angle concurrently; more batches shrink each step's

@@ -68,6 +68,34 @@ def test_groups_per_lane_bitwise(engine, precision, lagged):
         assert np.array_equal(ref[1], got[1])
 
 
+@pytest.mark.parametrize("engine", [ENGINE_LEVELS, ENGINE_HYBRID])
+@pytest.mark.parametrize("precision,ng", [(FP64, 1), (FP64, 8), (FP32, 4),
+                                          (FP32, 8), (FP32, 16), (FP32, 64)])
+@pytest.mark.parametrize("lagged", [False, True], ids=["acyclic", "lagged"])
+def test_two_items_per_lane_bitwise(engine, precision, ng, lagged):
+    """Two items per lane (both items' loads issued before either solve)
+    give one item per lane's results bit for bit, also on odd item counts
+    where the last wavefront holds one set."""
+    print(f"two items per lane engine={engine} precision={precision} "
+          f"ng={ng} lagged={lagged}", flush=True)
+    args = (problem(3, 2, ng, seed=3, jitter=0.2) if lagged
+            else problem(4, 4, ng, seed=44))
+    ref = run_steps(args, precision, engine)
+    sm, conn, omega, wt, st, ss, q = args
+    with SweeperHIP(sm, conn, omega, wt, ng, st, ss, precision) as s:
+        s.set_engine(engine, SPIN)
+        s.set_launch(max(1, 64 // ng), 1)
+        s.set_items_per_lane(2)
+        with pytest.raises(T2Error):
+            s.set_launch(0, 2)             # several groups per lane
+        s.set_source_isotropic(q)
+        for _ in range(3):
+            s.sweep_step()
+        got = (s.phi(), s.psi_angle(omega.shape[0] - 1))
+    assert np.array_equal(ref[0], got[0])
+    assert np.array_equal(ref[1], got[1])
+
+
 @pytest.mark.parametrize("precision,ng", [(FP64, 1), (FP64, 4), (FP64, 16),
                                           (FP32, 1), (FP32, 16)])
 def test_queue_matches_levels_bitwise(precision, ng):
@@ -300,6 +328,7 @@ def test_autotune_keeps_results_and_applies_best():
         assert inf["engine"] == best["engine"]
         assert inf["angle_batches"] == best["batches"]
         assert inf["groups_per_lane"] == best["groups_per_lane"]
+        assert any(t["items_per_lane"] == 2 for t in trials)
         assert inf["concurrent_batches"] == int(best["concurrent"])
         want = best.get("priority", (-1, 0, 0))
         assert (inf["intra_angle_p"], inf["inter_angle_p"],

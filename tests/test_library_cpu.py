@@ -133,6 +133,7 @@ def test_null_handle_errors():
     assert lib.t2_set_concurrent_batches(None, 1) == -1
     assert lib.t2_batch_set_concurrent(None, 1) == -1
     assert lib.t2_batch_set_launch(None, 64) == -1
+    assert lib.t2_set_items_per_lane(None, 2) == -1
     assert lib.t2_batch_solve_refined(None, 10, 1e-6, 1e-5, 10, None,
                                       None) == -1
     assert lib.t2_defect(None, None, None) == -1

@@ -24,6 +24,7 @@ BATCHES = (1, 2, 4, 8, 16)
 CHUNK_ROUNDS = (2, 4, 8)          # ready-queue chunk size, in rounds
 BLOCKS = (256, 128, 64)          # threads per block
 GROUPS_PER_LANE = (1, 2)
+GAUSS_NOPIVOT = 1
 CHAINS = (0, 1, 2, 4)             # batch level form: parallel chains
 
 
@@ -48,6 +49,7 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
     trials = []
     best = None
     s.set_engine(ENGINE_LEVELS)
+    s.set_items_per_lane(1)
     # schedule first (it rebuilds the records), with the default launch;
     # several angle batches one after the other or side by side
     for nb in batches:
@@ -59,7 +61,7 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
             s.set_concurrent_batches(conc)
             ms = _time_sweep(ev, s, reps)
             t = dict(engine=ENGINE_LEVELS, batches=nb, concurrent=conc,
-                     block=256, groups_per_lane=1, ms=ms)
+                     block=256, groups_per_lane=1, items_per_lane=1, ms=ms)
             trials.append(t)
             if log:
                 log(f"tune {t}")
@@ -78,7 +80,24 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
             ms = _time_sweep(ev, s, reps)
             t = dict(engine=ENGINE_LEVELS, batches=best["batches"],
                      concurrent=best["concurrent"], block=block,
-                     groups_per_lane=gpl, ms=ms)
+                     groups_per_lane=gpl, items_per_lane=1, ms=ms)
+            trials.append(t)
+            if log:
+                log(f"tune {t}")
+            if ms < best["ms"]:
+                best = t
+    # two items per lane (one group per lane, NoPivot)
+    if s.info()["gauss"] == GAUSS_NOPIVOT:
+        for block in blocks:
+            if block % ng:
+                continue
+            s.set_launch(block // ng, 1)
+            s.set_items_per_lane(2)
+            ms = _time_sweep(ev, s, reps)
+            s.set_items_per_lane(1)
+            t = dict(engine=ENGINE_LEVELS, batches=best["batches"],
+                     concurrent=best["concurrent"], block=block,
+                     groups_per_lane=1, items_per_lane=2, ms=ms)
             trials.append(t)
             if log:
                 log(f"tune {t}")
@@ -86,6 +105,7 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
                 best = t
     lanes = ng // best["groups_per_lane"]
     s.set_launch(best["block"] // lanes, best["groups_per_lane"])
+    s.set_items_per_lane(best["items_per_lane"])
     # the persistent engine at the best shape, over the schedules it may
     # prefer (fewer steps matter less to it)
     level_best = best
@@ -98,6 +118,7 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
                 t = dict(engine=engine, batches=nb, concurrent=False,
                          block=level_best["block"],
                          groups_per_lane=level_best["groups_per_lane"],
+                         items_per_lane=level_best["items_per_lane"],
                          chunk_rounds=rounds, ms=ms)
                 trials.append(t)
                 if log:
@@ -118,6 +139,7 @@ def tune(s, reps: int = 3, batches=BATCHES, blocks=BLOCKS,
             t = dict(engine=ENGINE_QUEUE, batches=1, concurrent=False,
                      block=level_best["block"],
                      groups_per_lane=level_best["groups_per_lane"],
+                     items_per_lane=level_best["items_per_lane"],
                      chunk_rounds=rounds, priority=tuple(rule), ms=ms)
             trials.append(t)
             if log:

@@ -47,6 +47,7 @@ struct t2_sweeper {
     int64_t nc = 0, na = 0, ng = 0;
     int precision = T2_FP64, gauss = T2_GAUSS_NOPIVOT;
     int block = 256, gpl = 1, batches = 1, order = T2_ORDER_FACES;
+    int ipl = 1;                     // items per lane (level engine)
     int intra = -1, inter = 0;       // priority schedule (intra < 0: levels)
     int64_t step_width = 0;
     bool use_graph = true;
@@ -253,6 +254,7 @@ int enqueue_sweep(t2_sweeper *s, const double *ang, const void *ang_t)
     a.psi = static_cast<T *>(s->d_psi);
     a.lag_buf = static_cast<const T *>(s->d_lag_buf);
     a.solver = cell_solver(s->gauss);
+    a.items_per_lane = s->ipl;
     const auto *geo = static_cast<const t2::ItemGeo<T> *>(s->d_item_geo);
     auto launch_steps = [&](int64_t lo, int64_t hi, hipStream_t stream) {
         for (int64_t st = lo; st < hi; st++) {
@@ -1258,6 +1260,9 @@ int t2_set_launch(t2_sweeper *s, int items_per_block, int groups_per_lane)
                                 "one group per lane");
     if (gpl > s->ng)
         return fail(T2_ERR_ARG, "groups_per_lane exceeds the group count");
+    if (gpl != 1 && s->ipl != 1)
+        return fail(T2_ERR_ARG, "several groups per lane need one item per "
+                                "lane (t2_set_items_per_lane 1)");
     int block = s->block;
     if (items_per_block) {
         const int64_t lanes = int64_t(items_per_block) * (s->ng / gpl);
@@ -1274,6 +1279,23 @@ int t2_set_launch(t2_sweeper *s, int items_per_block, int groups_per_lane)
     }
     s->block = block;
     s->gpl = gpl;
+    return T2_OK;
+}
+
+int t2_set_items_per_lane(t2_sweeper *s, int items)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (items != 1 && items != 2)
+        return fail(T2_ERR_ARG, "items per lane must be 1 or 2");
+    if (items != 1 && (s->gauss != T2_GAUSS_NOPIVOT || s->gpl != 1))
+        return fail(T2_ERR_ARG, "two items per lane need the NoPivot cell "
+                                "solver and one group per lane");
+    if (items == s->ipl)
+        return T2_OK;
+    T2_HIP(hipStreamSynchronize(s->stream));
+    drop_graphs(s);
+    s->ipl = items;
     return T2_OK;
 }
 

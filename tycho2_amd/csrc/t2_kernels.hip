@@ -419,6 +419,161 @@ sweep_step_kernel(const SweepArgs<T> p)
     solve_item<T, G, GPL, false, SOLVER>(p, item, gl, t);
 }
 
+// ------------------------------------------------------- two items per lane
+// Inputs of one item for one group, loaded ahead of its solve.
+template <typename T>
+struct ItemLoad {
+    T a12[4], v20, sig;
+    T q[4];
+    T uw[4][4];        // upwind values per face (0 where face k is not read)
+    int32_t up[4];
+};
+
+template <typename T, int G>
+__device__ __forceinline__ void load_item(const SweepArgs<T> &p, int64_t item,
+                                          int g, ItemLoad<T> &in)
+{
+    constexpr uint32_t CG = 4 * G;
+    const ItemTopo t = p.topo[item];
+    const ItemGeo<T> *gp = p.geo + item;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        in.a12[k] = gp->a12[k];
+        in.up[k] = t.up[k];
+    }
+    in.v20 = gp->v20;
+    in.sig = gp->sig;
+    if (p.sigma_mode != kSigmaCell)
+        in.sig = p.sigma[p.sigma_mode == kSigmaGroup
+                             ? uint32_t(g)
+                             : uint32_t(t.cell) * G + g];
+    const size_t plane = size_t(p.nc) * CG;
+    const T *psi_a = p.psi + size_t(t.angle) * plane;
+    const uint32_t own = uint32_t(t.cell) * CG;
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        in.q[v] = p.src_iso[own + v * G + g];
+    if (p.src_ang) {
+        const double *qa = p.src_ang + size_t(t.angle) * plane + own + g;
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            in.q[v] += T(qa[v * G]);
+    }
+    if (p.src_ang_t) {
+        const T *qa = p.src_ang_t + size_t(t.angle) * plane + own + g;
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            in.q[v] += qa[v * G];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            in.uw[k][j] = T(0);
+        const int32_t u = t.up[k];
+        if (u == -1)
+            continue;
+        if (u >= 0) {
+            const T *pn = psi_a + uint32_t(u) * CG + g;
+            int slot = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (j == k)
+                    continue;
+                const uint32_t vtx = (t.meta >> (6 * k + 2 * slot)) & 3u;
+                in.uw[k][j] = pn[vtx * G];
+                slot++;
+            }
+        } else {
+            const T *pl = p.lag_buf + size_t(-2 - u) * CG + g;
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (j != k)
+                    in.uw[k][j] = pl[j * G];
+        }
+    }
+}
+
+// The cell solve of solve_item on preloaded inputs (same arithmetic, same
+// order), and the store of psi.
+template <typename T, int G>
+__device__ __forceinline__ void solve_store_item(const SweepArgs<T> &p,
+                                                 int64_t item, int g,
+                                                 const ItemLoad<T> &in)
+{
+    constexpr uint32_t CG = 4 * G;
+    T o[4];
+    T P = T(0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        o[k] = in.a12[k] > T(0) ? in.a12[k] : T(0);
+        P += o[k];
+    }
+    const T s = in.sig * in.v20;
+    const T sumq = (in.q[0] + in.q[1]) + (in.q[2] + in.q[3]);
+    T b[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        b[v] = in.v20 * (sumq + in.q[v]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (in.up[k] == -1)
+            continue;
+        const T *uv = in.uw[k];
+        const T sum = (uv[0] + uv[1]) + (uv[2] + uv[3]);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (i != k)
+                b[i] -= in.a12[k] * (sum + uv[i]);
+    }
+    T m[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            m[i][j] = (i == j) ? in.a12[i] + T(2) * (s + (P - o[i]))
+                               : in.a12[i] + s + ((P - o[i]) - o[j]);
+    solve_nopivot<T>(m, b);
+    const ItemTopo t = p.topo[item];
+    T *dst = p.psi + size_t(t.angle) * size_t(p.nc) * CG +
+             uint32_t(t.cell) * CG + g;
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        __builtin_nontemporal_store(b[v], dst + v * G);
+}
+
+// Two items per lane: a wavefront covers items [w 2I, w 2I + 2I) with
+// I = 64 / G items per set, lane (i, g) solving group g of items w 2I + i and
+// w 2I + I + i.  Both items' loads are issued before either is solved and
+// stored (the compiler cannot move loads of psi past stores to psi), so a
+// wavefront has twice the loads in flight of one item per lane, each access
+// with one item per lane's pattern.  Single-problem launches, NoPivot.
+template <typename T, int G>
+__global__ void __launch_bounds__(kMaxBlock)
+sweep_step2_kernel(const SweepArgs<T> p)
+{
+    constexpr int I = 64 / G;
+    const int64_t per_xcd = gridDim.x / kXcds;
+    const int64_t lb =
+        int64_t(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+    const int64_t tid = lb * blockDim.x + threadIdx.x;
+    const int64_t wave = tid / 64;
+    const int lane = int(tid % 64);
+    const int g = lane % G;
+    const int64_t a = wave * 2 * I + lane / G;
+    const int64_t b = a + I;
+    if (a >= p.n_items)
+        return;
+    ItemLoad<T> in_a, in_b;
+    load_item<T, G>(p, a, g, in_a);
+    const bool has_b = b < p.n_items;
+    if (has_b)
+        load_item<T, G>(p, b, g, in_b);
+    solve_store_item<T, G>(p, a, g, in_a);
+    if (has_b)
+        solve_store_item<T, G>(p, b, g, in_b);
+}
+
 // ---------------------------------------------------------------- ready queue
 // The whole sweep in one persistent launch.  Blocks take chunks of work items
 // from a global ticket counter in step order (a chunk never spans two steps)
@@ -672,6 +827,18 @@ hipError_t launch_g(const SweepArgs<T> &args, int gpl, int block,
     const int64_t threads = args.n_items * lanes_per_item;
     const int64_t blocks = (threads + block - 1) / block;
     const unsigned grid = unsigned((blocks + kXcds - 1) / kXcds * kXcds);
+    if (args.items_per_lane == 2) {
+        if (gpl != 1 || args.solver != kSolveNoPivot || args.problems ||
+            args.active)
+            return hipErrorInvalidValue;
+        // a wavefront covers 2 x 64 / G items
+        const int64_t waves = (args.n_items * G + 127) / 128;
+        const int64_t blocks2 = (waves * 64 + block - 1) / block;
+        const unsigned grid2 = unsigned((blocks2 + kXcds - 1) / kXcds * kXcds);
+        hipLaunchKernelGGL((sweep_step2_kernel<T, G>), dim3(grid2),
+                           dim3(block), 0, stream, args);
+        return hipGetLastError();
+    }
     if (args.solver != kSolveNoPivot) {
         // the other solvers run one group per lane
         if (gpl != 1)

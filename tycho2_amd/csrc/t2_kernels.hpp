@@ -83,6 +83,10 @@ struct SweepArgs {
     // CellSolver; the level engine's kernels take any, the ready-queue
     // kernels kSolveNoPivot
     int solver;
+    // level engine, kSolveNoPivot, one group per lane, one problem (no
+    // problems / active tables): 2 solves two items per lane, their loads
+    // issued before either solve (0 or 1: one item per lane)
+    int items_per_lane;
 };
 
 // Defect of the coupled transport equations for a stored angular flux,

@@ -61,6 +61,8 @@ SIGNATURES = {
                                      ctypes.c_int]),
     "t2_set_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "t2_set_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "t2_set_items_per_lane": (ctypes.c_int, [ctypes.c_void_p,
+                                             ctypes.c_int]),
     "t2_set_concurrent_batches": (ctypes.c_int, [ctypes.c_void_p,
                                                  ctypes.c_int]),
     "t2_set_priority": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
@@ -308,6 +310,9 @@ class SweeperHIP:
     def set_launch(self, items_per_block: int = 0, groups_per_lane: int = 0):
         _check(lib().t2_set_launch(self.handle, items_per_block,
                                    groups_per_lane))
+
+    def set_items_per_lane(self, items: int) -> None:
+        _check(lib().t2_set_items_per_lane(self.handle, int(items)))
 
     def set_schedule(self, angle_batches: int) -> None:
         _check(lib().t2_set_schedule(self.handle, int(angle_batches)))
