@@ -250,14 +250,19 @@ int t2_set_source_angular(t2_sweeper *s, const double *source);
 
 /* Source iteration on the device (src/SourceIteration.cc:127-207): starts
  * from psi = 0, stops when the L1-relative phi change <= err_max or after
- * iter_max sweeps.  err_hist (host, length iter_max) may be NULL. */
+ * iter_max sweeps.  err_hist (host, length iter_max) may be NULL.
+ * Returns T2_ERR_NOCONV when iter_max stopped it above err_max (the
+ * reference stops there silently); phi, psi and *n_iter then hold the last
+ * iterate. */
 int t2_solve_source_iteration(t2_sweeper *s, int64_t iter_max, double err_max,
                               int64_t *n_iter, double *err_hist);
 
 /* GMRES(restart) on (I - D L^-1 M S) phi = D L^-1 q with x0 = 0, then one
  * sweep for psi (src/SourceIteration.cc:216-285).  Arnoldi, Gram-Schmidt and
  * the iterate are FP64 on the device whatever the sweep precision.
- * res_hist (host, length iter_max) may be NULL. */
+ * res_hist (host, length iter_max) may be NULL.  Returns T2_ERR_NOCONV when
+ * iter_max operator applies left the relative residual above rtol (phi and
+ * psi hold the last iterate). */
 int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,
                     int64_t restart, int64_t *n_iter, double *res_hist);
 
@@ -291,11 +296,16 @@ int t2_krylov_apply_device(t2_sweeper *s, const double *x, double *b);
  *      apply, no sweep), stored in FP32 as a nodal source density.
  *   3. GMRES on the correction (I - D L^-1 M S) D delta = D L^-1 r with FP32
  *      sweeps, to rtol_correction; phi = phi1 + D delta in FP64.
- *   4. psi = L^-1 (q + M S phi) by one FP32 sweep (the angular flux that
- *      goes with phi, to FP32 accuracy).
+ *   4. psi = psi32 + delta: psi32 is swept again (step 3's sweeps reused its
+ *      buffer), and delta = L^-1 (r + M S D delta) by one FP32 sweep written
+ *      over the residual.  t2_get_psi / t2_get_psi_angle then return
+ *      psi32 + delta added in FP64: the angular flux to the accuracy of phi,
+ *      not of FP32, until the next sweep or t2_defect replaces either part.
  * Memory: psi32 plus an FP32 residual (psi-sized each).  phi1 (host,
  * [cell][vertex][group], may be NULL) receives the scalar flux after step 1
- * so callers can report the error of each step. */
+ * so callers can report the error of each step.  Every step runs; the call
+ * returns T2_ERR_NOCONV if either GMRES stage stopped at iter_max above its
+ * tolerance (info holds both stages' residuals). */
 typedef struct {
     int64_t iters_sweep;       /* GMRES iterations of step 1              */
     int64_t iters_correction;  /* GMRES iterations of step 3              */
@@ -374,7 +384,8 @@ int t2_batch_info(const t2_batch *b, int64_t *n_steps, int64_t *n_items);
  * SI step) shares one batched sweep; finished members drop out of the sweeps.
  * Each member's phi and psi equal its own solve's bit for bit.  Per member
  * (arrays of the member count, may be NULL): iterations, and the wall time
- * from the call to the member's converged phi (seconds). */
+ * from the call to the member's converged phi (seconds).  T2_ERR_NOCONV when
+ * any member stopped at iter_max above its tolerance (outputs are filled). */
 int t2_batch_solve_krylov(t2_batch *b, int64_t iter_max, double rtol,
                           int64_t restart, int64_t *n_iter, double *seconds);
 int t2_batch_solve_source_iteration(t2_batch *b, int64_t iter_max,
@@ -382,16 +393,27 @@ int t2_batch_solve_source_iteration(t2_batch *b, int64_t iter_max,
                                     double *seconds);
 /* t2_solve_refined of every FP32 member, each of its stages in lock step
  * (the correction's right-hand side sweep carries each member's own FP32
- * residual source).  info and seconds (arrays of the member count, may be
- * NULL): each member's t2_refine_info, and the wall time from the call to
- * the end of its correction GMRES. */
+ * residual source, and the final correction sweep writes each member's
+ * residual buffer in place).  info and seconds (arrays of the member count,
+ * may be NULL): each member's t2_refine_info, and the wall time from the
+ * call to the end of its correction GMRES.  T2_ERR_NOCONV as for one
+ * handle, when any member's stage stopped above its tolerance. */
 int t2_batch_solve_refined(t2_batch *b, int64_t iter_max, double rtol_sweep,
                            double rtol_correction, int64_t restart,
                            t2_refine_info *info, double *seconds);
 int t2_batch_synchronize(t2_batch *b);
 int t2_batch_get_stream(const t2_batch *b, void **stream);
 
-/* Copy results to the host in the reference layouts. */
+/* Testing hook for the out-of-memory paths: the nth device allocation of a
+ * sweeper from now on (nth >= 1, counted over all handles) fails with
+ * T2_ERR_NOMEM as hipMalloc running out of memory would; 0 disables it.
+ * Everything allocated before the failure is released by the failing call
+ * (creation) or stays owned by its handle.  No reference counterpart (the
+ * reference aborts on any failure, src/Assert.cc:51-65). */
+int t2_set_alloc_fail(int64_t nth);
+
+/* Copy results to the host in the reference layouts (after a refined
+ * solve, psi is psi32 + its FP32 correction added in FP64). */
 int t2_get_phi(const t2_sweeper *s, double *phi);
 int t2_get_psi(const t2_sweeper *s, double *psi);
 /* One angle of psi, psi_a[cell][vertex][group] (e.g. to inspect full-size

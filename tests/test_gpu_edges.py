@@ -93,3 +93,106 @@ def test_sixty_four_groups():
     src = rng.uniform(-0.5, 2.0, (sm.n_cells, omega.shape[0], 4, 64))
     psi, ref = sweep_both(sm, omega, wt, 64, st, src)
     assert maxrel(psi, ref) <= 1e-12
+
+
+def zero_face_directions(sm, m):
+    """Directions with Omega . n exactly zero on some faces of an unjittered
+    Kuhn cube: two in the xy-plane (zero on every z-normal face) and two
+    along Omega_x = Omega_y (zero on the faces whose normal has n_x =
+    -n_y, n_z = 0)."""
+    c, s = 0.6, 0.8
+    d = 1.0 / np.sqrt(3.0)
+    omega = np.array([[c, s, 0.0], [-s, c, 0.0], [d, d, d], [-d, -d, d]])
+    wt = np.full(4, np.pi)
+    dots = np.einsum("ad,ckd->ack", omega, m.normal)
+    return omega, wt, dots
+
+
+def test_exactly_zero_omega_dot_n_faces():
+    """Caller-supplied directions perpendicular to face normals: the
+    traversal counts an Omega . n = 0 face as incoming on both sides
+    (src/TychoMesh.hh:107-110), which makes a cycle that is broken by
+    lagging it; the cell solve gives it no coupling (A (Omega . n) = 0,
+    src/Transport.cc:113,184).  Device and oracle agree, whatever psi the
+    lagged faces read."""
+    print("exactly zero Omega.n faces", flush=True)
+    sm = meshmod.kuhn_cube(3, 0.0)
+    m = orc.build_mesh(sm)
+    omega, wt, dots = zero_face_directions(sm, m)
+    interior = m.adj >= 0
+    zeros = [(dots[a] == 0.0)[interior].sum() for a in range(4)]
+    print(f"  exact-zero interior faces per direction: {zeros}")
+    assert all(z > 0 for z in zeros)
+    ng = 4
+    rng = np.random.default_rng(11)
+    st = rng.uniform(0.5, 2.0, (sm.n_cells, ng))
+    src = rng.uniform(0.0, 1.0, (sm.n_cells, 4, 4, ng))
+    prev = rng.uniform(0.0, 1.0, src.shape)
+    conn = meshmod.tycho_connectivity(sm)
+    psi = prev.copy()
+    with SweeperHIP(sm, conn, omega, wt, ng, st, 0.5 * st, FP64) as s:
+        n_lag = s.info()["n_lagged"]
+        s.sweep(psi, src)
+        # the lagged faces' previous psi has no effect: a sweep from zeros
+        psi0 = np.zeros_like(src)
+        s.sweep(psi0, src)
+    ref, lag = orc.sweep(m, omega, ng, st, src, psi=prev.copy())
+    print(f"  lagged faces: engine {n_lag}, oracle {lag}")
+    assert n_lag == lag > 0
+    assert maxrel(psi, ref) <= 1e-12
+    assert np.array_equal(psi, psi0)
+
+
+def hip_free_bytes():
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return free.value
+
+
+@pytest.mark.parametrize("precision,lagged", [(FP64, True), (FP32, False)])
+def test_device_allocation_failures_are_clean(precision, lagged):
+    """Every device allocation of a handle's creation failed in turn by the
+    testing hook (t2_set_alloc_fail): creation returns T2_ERR_NOMEM and
+    releases what it had taken, and the next creation succeeds and sweeps
+    as the oracle does."""
+    from tycho2_amd.sweeper import T2Error, set_alloc_fail
+    print(f"allocation failures precision={precision} lagged={lagged}",
+          flush=True)
+    if lagged:   # S2 along the Kuhn diagonal faces: lagged faces
+        sm = meshmod.kuhn_cube(3, 0.2, seed=3)
+        omega, wt = orc.quadrature(2)
+    else:
+        sm = meshmod.kuhn_cube(3, 0.15, seed=3)
+        omega, wt = product_quadrature(4)
+    conn = meshmod.tycho_connectivity(sm)
+    ng = 2
+    rng = np.random.default_rng(12)
+    st = rng.uniform(0.5, 2.0, (sm.n_cells, ng))
+    with SweeperHIP(sm, conn, omega, wt, ng, st, 0.5 * st, precision):
+        pass                                   # warm the runtime's pools
+    free0 = hip_free_bytes()
+    failures = 0
+    try:
+        for nth in range(1, 200):
+            set_alloc_fail(nth)
+            try:
+                s = SweeperHIP(sm, conn, omega, wt, ng, st, 0.5 * st,
+                               precision)
+            except T2Error as e:
+                assert e.code == -3 and "t2_set_alloc_fail" in str(e), e
+                failures += 1
+                continue
+            s.close()
+            break
+        else:
+            pytest.fail("creation never succeeded")
+    finally:
+        set_alloc_fail(0)
+    print(f"  {failures} allocations of a creation failed in turn")
+    assert failures >= 10
+    assert abs(hip_free_bytes() - free0) <= 64 << 20
+    src = rng.uniform(0.0, 1.0, (sm.n_cells, omega.shape[0], 4, ng))
+    psi, ref = sweep_both(sm, omega, wt, ng, st, src, precision=precision)
+    assert maxrel(psi, ref) <= (1e-12 if precision == FP64 else 2e-5)

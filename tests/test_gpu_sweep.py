@@ -270,16 +270,75 @@ def test_refined_solve_reaches_1e10(ratio, bar):
     q = np.ones((sm.n_cells, 4, ng))
     src = orc.isotropic_source(sm.n_cells, omega.shape[0], ng,
                                np.ones((sm.n_cells, ng)))
-    _, rphi, _ = orc.krylov(orc.build_mesh(sm), omega, wt, ng, st, ss, src,
-                            400, 1e-14, 30)
+    rpsi, rphi, _ = orc.krylov(orc.build_mesh(sm), omega, wt, ng, st, ss,
+                               src, 400, 1e-14, 30)
     with SweeperHIP(sm, conn, omega, wt, ng, st, ss, FP32) as s:
         s.set_source_isotropic(q)
         info, phi1 = s.solve_refined(300, 1e-6, 1e-5, 20, want_phi1=True)
         phi = s.phi()
+        psi = s.psi()
+        psi_a = s.psi_angle(5)
+        # a plain sweep afterwards replaces psi: FP32 accuracy again
+        s.sweep_step()
+        psi32 = s.psi()
     e1, e2 = maxrel(phi1, rphi), maxrel(phi, rphi)
-    print(f"refined: FP32 stage {e1:.2e}, after correction {e2:.2e}, {info}")
+    ep, ep32 = maxrel(psi, rpsi), maxrel(psi32, rpsi)
+    print(f"refined: FP32 stage {e1:.2e}, after correction {e2:.2e}, "
+          f"psi {ep:.2e} (a plain FP32 sweep: {ep32:.2e}), {info}")
     assert e2 <= bar
     assert e1 > 100 * e2
+    # psi = psi32 + its FP32 correction meets the phi bar too; the two
+    # accessors agree
+    assert ep <= bar
+    assert ep32 > 100 * ep
+    assert np.array_equal(psi_a, psi[:, 5])
+    # right-hand sides 2, psi32 twice, the correction 1, and one residual
+    # apply per GMRES(20) restart
+    restarts = (info["iters_sweep"] - 1) // 20 + \
+        (info["iters_correction"] - 1) // 20
+    assert info["sweeps"] == \
+        info["iters_sweep"] + info["iters_correction"] + 5 + restarts
+
+
+def test_iteration_limits_report_noconv():
+    """Every solve that stops at its iteration limit above its tolerance
+    returns T2_ERR_NOCONV with the last iterate in place (and the call's
+    results on the exception); one that converges returns T2_OK."""
+    from tycho2_amd.sweeper import NoConvergence, SweepBatch
+    print("iteration limits", flush=True)
+    ng = 4
+    sm, conn, omega, wt, st, ss, rng = random_problem(n=3, ng=ng, seed=41)
+    ss = 0.9 * st
+    q = rng.uniform(0.5, 1.5, (sm.n_cells, 4, ng))
+    with SweeperHIP(sm, conn, omega, wt, ng, st, ss, FP64) as s:
+        s.set_source_isotropic(q)
+        with pytest.raises(NoConvergence) as e:
+            s.solve_source_iteration(3, 1e-14)
+        assert e.value.code == -6 and len(e.value.result) == 3
+        assert e.value.result[-1] > 1e-14
+        with pytest.raises(NoConvergence) as e:
+            s.solve_krylov(2, 1e-14, 20)
+        assert len(e.value.result) == 2
+        assert len(s.solve_krylov(200, 1e-12, 20)) < 200
+    with SweeperHIP(sm, conn, omega, wt, ng, st, ss, FP32) as s:
+        s.set_source_isotropic(q)
+        # FP32 sweeps cannot take the first stage to 1e-12
+        with pytest.raises(NoConvergence) as e:
+            s.solve_refined(6, 1e-12, 1e-5, 20)
+        info, _ = e.value.result
+        assert info["iters_sweep"] == 6 and info["residual_sweep"] > 1e-12
+    mem = [SweeperHIP(sm, conn, omega, wt, ng, st, ss, FP64) for _ in range(2)]
+    for m in mem:
+        m.set_source_isotropic(q)
+    with SweepBatch(mem) as b:
+        with pytest.raises(NoConvergence) as e:
+            b.solve_krylov(2, 1e-14, 10)
+        its, _ = e.value.result
+        assert list(its) == [2, 2]
+        with pytest.raises(NoConvergence):
+            b.solve_source_iteration(2, 1e-14)
+    for m in mem:
+        m.close()
 
 
 def batch_members(precision, groups=(4, 4, 4)):

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -32,10 +33,13 @@ using t2::fail;
             return rc_;     \
     } while (0)
 
-// A captured sweep, keyed by the angle-dependent sources it reads.
+// A captured sweep, keyed by the angle-dependent sources it reads and the
+// angular-flux buffer it writes.
 struct GraphEntry {
     const double *ang;     // FP64 angle-dependent source or null
     const void *ang_t;     // psi-precision angle-dependent source or null
+    const void *out;       // angular flux written (the handle's psi or the
+                           // refined solve's correction buffer)
     hipGraphExec_t exec;
 };
 
@@ -81,7 +85,9 @@ struct t2_sweeper {
     void *d_psi = nullptr;           // T [na][nc][4][G]
     void *d_src = nullptr;           // T [nc][4][G]
     double *d_src64 = nullptr;       // FP64 [nc][4][G] (residual)
-    float *d_res = nullptr;          // FP32 [na][nc][4][G] (residual)
+    float *d_res = nullptr;          // FP32 [na][nc][4][G] (residual; after
+                                     // a refined solve, the correction of psi)
+    bool psi_corrected = false;      // psi = d_psi + d_res (refined solve)
     double *d_qiso = nullptr;        // FP64 [nc][4][G]
     double *d_qang = nullptr;        // FP64 [na][nc][4][G]
     double *d_stage = nullptr;       // FP64 psi-sized staging (host path)
@@ -119,11 +125,19 @@ struct t2_sweeper {
 
 namespace {
 
+// Allocation-failure hook (t2_set_alloc_fail): the device allocation that
+// brings the count down to zero fails as an out-of-memory would.
+std::atomic<int64_t> g_alloc_fail{0};
+
 int dalloc(t2_sweeper *s, void **p, size_t bytes)
 {
     *p = nullptr;
     if (bytes == 0)
         bytes = 8;
+    if (g_alloc_fail.load(std::memory_order_relaxed) > 0 &&
+        --g_alloc_fail == 0)
+        return fail(T2_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) +
+                                      " B): failed by t2_set_alloc_fail");
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
@@ -235,15 +249,24 @@ int ensure_side_streams(t2_sweeper *s, int64_t n)
     return T2_OK;
 }
 
+// psi_out = L^-1 (src + ang + ang_t).  Lagged faces read a snapshot of the
+// handle's psi taken before the sweep; a sweep into another buffer (the
+// refined solve's correction, whose previous value is zero) reads zeros
+// there.
 template <typename T>
-int enqueue_sweep(t2_sweeper *s, const double *ang, const void *ang_t)
+int enqueue_sweep(t2_sweeper *s, const double *ang, const void *ang_t,
+                  void *psi_out)
 {
     const int64_t n_lag = int64_t(s->sched.lagged.size());
-    if (n_lag > 0)
+    if (n_lag > 0 && psi_out == s->d_psi)
         T2_HIP(t2::launch_lag_gather<T>(
             s->d_lag_key, s->d_lag_up, n_lag, s->nc, int(s->ng), s->d_nbr,
             static_cast<const T *>(s->d_psi), static_cast<T *>(s->d_lag_buf),
             s->stream));
+    else if (n_lag > 0)   // a whole number of doubles: 4 G values per face
+        T2_HIP(t2::launch_fill(static_cast<double *>(s->d_lag_buf), 0.0,
+                               int64_t(sizeof(T)) * n_lag * 4 * s->ng / 8,
+                               s->stream));
     t2::SweepArgs<T> a{};
     a.nc = s->nc;
     a.sigma_mode = s->sigma_mode;
@@ -251,7 +274,7 @@ int enqueue_sweep(t2_sweeper *s, const double *ang, const void *ang_t)
     a.src_iso = static_cast<const T *>(s->d_src);
     a.src_ang = ang;
     a.src_ang_t = static_cast<const T *>(ang_t);
-    a.psi = static_cast<T *>(s->d_psi);
+    a.psi = static_cast<T *>(psi_out);
     a.lag_buf = static_cast<const T *>(s->d_lag_buf);
     a.solver = cell_solver(s->gauss);
     a.items_per_lane = s->ipl;
@@ -317,10 +340,12 @@ int enqueue_sweep(t2_sweeper *s, const double *ang, const void *ang_t)
     return T2_OK;
 }
 
-int enqueue_sweep_any(t2_sweeper *s, const double *ang, const void *ang_t)
+int enqueue_sweep_any(t2_sweeper *s, const double *ang, const void *ang_t,
+                      void *psi_out)
 {
-    return s->precision == T2_FP64 ? enqueue_sweep<double>(s, ang, ang_t)
-                                   : enqueue_sweep<float>(s, ang, ang_t);
+    return s->precision == T2_FP64
+               ? enqueue_sweep<double>(s, ang, ang_t, psi_out)
+               : enqueue_sweep<float>(s, ang, ang_t, psi_out);
 }
 
 int queue_resident(t2_sweeper *s, int64_t *grid)
@@ -503,8 +528,13 @@ int check_step_launches(int64_t launches)
     return T2_OK;
 }
 
-int run_sweep(t2_sweeper *s, const double *ang, const void *ang_t = nullptr)
+int run_sweep(t2_sweeper *s, const double *ang, const void *ang_t = nullptr,
+              void *psi_out = nullptr)
 {
+    if (!psi_out)
+        psi_out = s->d_psi;
+    if (psi_out == s->d_psi)   // psi is replaced: no correction applies
+        s->psi_corrected = false;
     T2_TRY(ensure_queue(s));
     T2_TRY(check_step_launches(s->engine == T2_ENGINE_LEVELS
                                    ? s->sched.n_steps
@@ -514,16 +544,16 @@ int run_sweep(t2_sweeper *s, const double *ang, const void *ang_t = nullptr)
             s, int64_t(s->sched.batch_steps.size()) - 2));
     s->n_sweeps++;
     if (!s->use_graph)
-        return enqueue_sweep_any(s, ang, ang_t);
+        return enqueue_sweep_any(s, ang, ang_t, psi_out);
     hipGraphExec_t exec = nullptr;
     for (auto &g : s->graphs)
-        if (g.ang == ang && g.ang_t == ang_t)
+        if (g.ang == ang && g.ang_t == ang_t && g.out == psi_out)
             exec = g.exec;
     if (!exec) {
         hipGraph_t g = nullptr;
         T2_HIP(hipStreamBeginCapture(s->stream,
                                      hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_sweep_any(s, ang, ang_t);
+        int rc = enqueue_sweep_any(s, ang, ang_t, psi_out);
         hipError_t e = hipStreamEndCapture(s->stream, &g);
         if (rc != T2_OK) {
             if (g)
@@ -534,7 +564,7 @@ int run_sweep(t2_sweeper *s, const double *ang, const void *ang_t = nullptr)
         e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         T2_HIP(e);
-        s->graphs.push_back({ang, ang_t, exec});
+        s->graphs.push_back({ang, ang_t, psi_out, exec});
     }
     T2_HIP(hipGraphLaunch(exec, s->stream));
     return T2_OK;
@@ -1095,11 +1125,13 @@ struct Gmres {
     }
 
     double rel_res() const { return rnorm / (bnorm > 0 ? bnorm : 1.0); }
+    // false when the iteration limit stopped the solve above its target
+    bool converged() const { return rnorm <= target; }
 };
 
 int gmres(t2_sweeper *s, const double *b, double *x, int64_t m, double rtol,
           int64_t iter_max, double *V, int64_t *n_iter, double *hist,
-          double *rel_res)
+          double *rel_res, bool *converged)
 {
     Gmres gm;
     T2_TRY(gm.init(s, b, x, m, rtol, iter_max, V, hist));
@@ -1114,7 +1146,22 @@ int gmres(t2_sweeper *s, const double *b, double *x, int64_t m, double rtol,
         *n_iter = gm.it;
     if (rel_res)
         *rel_res = gm.rel_res();
+    if (converged)
+        *converged = gm.converged();
     return T2_OK;
+}
+
+// Status of a solve that ran to its end: T2_ERR_NOCONV (with the stage's
+// figures in the message) when its iteration limit stopped it above its
+// tolerance.  The solution arrays hold the last iterate either way.
+int noconv(const std::string &what, int64_t iters, double reached,
+           double wanted)
+{
+    char buf[160];
+    std::snprintf(buf, sizeof(buf),
+                  ": %lld iterations reached %.3e, above the tolerance %.3e",
+                  (long long)iters, reached, wanted);
+    return fail(T2_ERR_NOCONV, what + buf);
 }
 
 }  // namespace
@@ -1186,6 +1233,15 @@ int t2_sweeper_create(t2_sweeper **out, int device, int64_t n_cells,
         return fail(rc, msg);
     }
     *out = s;
+    return T2_OK;
+}
+
+int t2_set_alloc_fail(int64_t nth)
+{
+    t2::clear_error();
+    if (nth < 0)
+        return fail(T2_ERR_ARG, "t2_set_alloc_fail: nth < 0");
+    g_alloc_fail.store(nth);
     return T2_OK;
 }
 
@@ -1481,6 +1537,8 @@ static int solve_si_body(t2_sweeper *s, int64_t iter_max, double err_max,
     }
     if (n_iter)
         *n_iter = it;
+    if (err > err_max)
+        return noconv("source iteration", it, err, err_max);
     return T2_OK;
 }
 
@@ -1499,12 +1557,19 @@ static int solve_krylov_body(t2_sweeper *s, int64_t iter_max, double rtol,
     T2_TRY(total_source(s, s->d_qiso, nullptr));
     T2_TRY(run_sweep(s, s->d_qang));
     T2_TRY(psi_to_phi(s, b));
-    T2_TRY(gmres(s, b, s->d_phi, restart, rtol, iter_max, V, n_iter,
-                 res_hist, nullptr));
+    bool conv = true;
+    int64_t its = 0;
+    double rel = 0.0;
+    T2_TRY(gmres(s, b, s->d_phi, restart, rtol, iter_max, V, &its, res_hist,
+                 &rel, &conv));
+    if (n_iter)
+        *n_iter = its;
     // psi = L^-1 (q + M S phi)
     T2_TRY(total_source(s, s->d_qiso, s->d_phi));
     T2_TRY(run_sweep(s, s->d_qang));
     T2_HIP(hipStreamSynchronize(s->stream));
+    if (!conv)
+        return noconv("GMRES", its, rel, rtol);
     return T2_OK;
 }
 
@@ -1534,8 +1599,9 @@ static int solve_refined_body(t2_sweeper *s, int64_t iter_max, double rtol_sweep
     T2_TRY(total_source(s, s->d_qiso, nullptr));
     T2_TRY(run_sweep(s, s->d_qang));
     T2_TRY(psi_to_phi(s, b));
+    bool conv1 = true, conv2 = true;
     T2_TRY(gmres(s, b, s->d_phi, restart, rtol_sweep, iter_max, V,
-                 &st.iters_sweep, nullptr, &st.residual_sweep));
+                 &st.iters_sweep, nullptr, &st.residual_sweep, &conv1));
     T2_TRY(total_source(s, s->d_qiso, s->d_phi));
     T2_TRY(run_sweep(s, s->d_qang));
     double *phi_1 = s->d_phi_old;
@@ -1566,8 +1632,20 @@ static int solve_refined_body(t2_sweeper *s, int64_t iter_max, double rtol_sweep
     T2_TRY(run_sweep(s, nullptr, s->d_res));
     T2_TRY(psi_to_phi(s, b));
     T2_TRY(gmres(s, b, y, restart, rtol_correction, iter_max, V,
-                 &st.iters_correction, nullptr, &st.residual_correction));
-    // y is D delta to the correction's tolerance: phi = phi1 + y
+                 &st.iters_correction, nullptr, &st.residual_correction,
+                 &conv2));
+    // 4. psi to FP64 accuracy, psi = psi32 + dpsi.  The correction's
+    // sweeps used the psi buffer, so psi32 = L^-1 (q + M S phi) is swept
+    // again from step 1's phi (bitwise the psi32 of the defect); then
+    // dpsi = L^-1 (r + M S D delta), y being D delta to the correction's
+    // tolerance, is swept in FP32 over the residual in place (each item reads
+    // its own r before it stores its dpsi, and upwind reads see dpsi).
+    T2_TRY(total_source(s, s->d_qiso, s->d_phi));
+    T2_TRY(run_sweep(s, s->d_qang));
+    T2_TRY(total_source(s, nullptr, y));
+    T2_TRY(run_sweep(s, nullptr, s->d_res, s->d_res));
+    s->psi_corrected = true;
+    // phi = phi1 + y
     double corr = 0.0, base = 0.0;
     T2_TRY(dot(s, y, y, n, &corr));
     T2_HIP(t2::launch_axpby(1.0, phi_1, 1.0, y, n, s->stream));
@@ -1575,13 +1653,17 @@ static int solve_refined_body(t2_sweeper *s, int64_t iter_max, double rtol_sweep
                           hipMemcpyDeviceToDevice, s->stream));
     T2_TRY(dot(s, s->d_phi, s->d_phi, n, &base));
     st.correction_rel = base > 0 ? std::sqrt(corr / base) : 0.0;
-    // psi consistent with the final phi (FP32 storage)
-    T2_TRY(total_source(s, s->d_qiso, s->d_phi));
-    T2_TRY(run_sweep(s, s->d_qang));
     T2_HIP(hipStreamSynchronize(s->stream));
     st.sweeps = s->n_sweeps - sweeps0;
     if (info)
         *info = st;
+    if (!conv1)
+        return noconv("refined solve, FP32-sweep GMRES stage", st.iters_sweep,
+                      st.residual_sweep, rtol_sweep);
+    if (!conv2)
+        return noconv("refined solve, correction GMRES stage",
+                      st.iters_correction, st.residual_correction,
+                      rtol_correction);
     return T2_OK;
 }
 
@@ -1623,10 +1705,10 @@ int t2_get_psi(const t2_sweeper *s_const, double *psi)
         T2_HIP(t2::launch_ac_to_ca<double>(
             static_cast<const double *>(s->d_psi), s->d_stage, s->nc, s->na,
             4 * s->ng, s->stream));
-    else
-        T2_HIP(t2::launch_ac_to_ca<float>(static_cast<const float *>(s->d_psi),
-                                          s->d_stage, s->nc, s->na, 4 * s->ng,
-                                          s->stream));
+    else   // after a refined solve: psi32 + its FP32 correction, in FP64
+        T2_HIP(t2::launch_ac_to_ca<float>(
+            static_cast<const float *>(s->d_psi), s->d_stage, s->nc, s->na,
+            4 * s->ng, s->stream, s->psi_corrected ? s->d_res : nullptr));
     return download_host(s, psi, s->d_stage, sizeof(double) * n);
 }
 
@@ -1647,6 +1729,8 @@ int t2_defect(t2_sweeper *s, const double *psi, double *density)
         T2_TRY(dalloc_t(s, &s->d_hostang, sizeof(double) * n));
     if (!fp64 && !s->d_res)
         T2_TRY(dalloc_t(s, &s->d_res, sizeof(float) * n));
+    if (!fp64)   // the residual buffer is overwritten
+        s->psi_corrected = false;
     if (psi) {
         T2_TRY(upload_host(s, s->d_stage, psi, sizeof(double) * n));
         if (fp64)
@@ -1770,12 +1854,17 @@ int t2_get_psi_angle(const t2_sweeper *s, int64_t angle, double *psi_a)
         T2_HIP(hipMemcpy(psi_a, static_cast<const double *>(s->d_psi) + angle * n,
                          sizeof(double) * n, hipMemcpyDeviceToHost));
     } else {
-        std::vector<float> tmp(n);
+        std::vector<float> tmp(n), corr;
         T2_HIP(hipMemcpy(tmp.data(),
                          static_cast<const float *>(s->d_psi) + angle * n,
                          sizeof(float) * n, hipMemcpyDeviceToHost));
+        if (s->psi_corrected) {   // psi32 + its correction, in FP64
+            corr.resize(n);
+            T2_HIP(hipMemcpy(corr.data(), s->d_res + angle * n,
+                             sizeof(float) * n, hipMemcpyDeviceToHost));
+        }
         for (int64_t i = 0; i < n; i++)
-            psi_a[i] = double(tmp[i]);
+            psi_a[i] = double(tmp[i]) + (corr.empty() ? 0.0 : double(corr[i]));
     }
     return T2_OK;
 }
@@ -1917,6 +2006,12 @@ struct t2_batch {
     std::vector<int32_t> active;         // host copy (kept alive for copies)
     std::vector<int32_t> uploaded;       // flags the device holds
     hipGraphExec_t graph = nullptr;
+    // correction sweeps of the refined solve: each member's sweep writes its
+    // residual buffer in place (its angle-dependent source) and lagged faces
+    // read zero; the tables point there while corr is set, and the sweep is
+    // captured in a graph of its own
+    bool corr = false;
+    hipGraphExec_t graph_corr = nullptr;
     std::vector<std::vector<const void *>> snap;  // members' buffers
 };
 
@@ -1989,11 +2084,21 @@ void drop_batch_graph(t2_batch *b)
 {
     if (b->graph)
         (void)hipGraphExecDestroy(b->graph);
+    if (b->graph_corr)
+        (void)hipGraphExecDestroy(b->graph_corr);
     b->graph = nullptr;
+    b->graph_corr = nullptr;
+}
+
+// Angular flux a member's batched sweep writes.
+void *member_psi_out(const t2_batch *b, const t2_sweeper *m)
+{
+    return b->corr ? static_cast<void *>(m->d_res) : m->d_psi;
 }
 
 template <typename T>
-t2::ProblemDesc<T> problem_desc(const t2_sweeper *m, const void *ang_t)
+t2::ProblemDesc<T> problem_desc(const t2_sweeper *m, const void *ang_t,
+                                void *psi_out)
 {
     t2::ProblemDesc<T> d{};
     d.nc = m->nc;
@@ -2001,13 +2106,14 @@ t2::ProblemDesc<T> problem_desc(const t2_sweeper *m, const void *ang_t)
     d.sigma = static_cast<const T *>(m->d_sigt);
     d.src_iso = static_cast<const T *>(m->d_src);
     d.src_ang_t = static_cast<const T *>(ang_t);
-    d.psi = static_cast<T *>(m->d_psi);
+    d.psi = static_cast<T *>(psi_out);
     d.lag_buf = static_cast<const T *>(m->d_lag_buf);
     return d;
 }
 
 template <typename T>
-t2::BatchProblem<T> batch_problem(const t2_sweeper *m, const void *ang_t)
+t2::BatchProblem<T> batch_problem(const t2_sweeper *m, const void *ang_t,
+                                  void *psi_out)
 {
     t2::BatchProblem<T> bp{};
     bp.a.geo = static_cast<const t2::ItemGeo<T> *>(m->d_item_geo);
@@ -2018,7 +2124,7 @@ t2::BatchProblem<T> batch_problem(const t2_sweeper *m, const void *ang_t)
     bp.a.sigma = static_cast<const T *>(m->d_sigt);
     bp.a.src_iso = static_cast<const T *>(m->d_src);
     bp.a.src_ang_t = static_cast<const T *>(ang_t);
-    bp.a.psi = static_cast<T *>(m->d_psi);
+    bp.a.psi = static_cast<T *>(psi_out);
     bp.a.lag_buf = static_cast<const T *>(m->d_lag_buf);
     bp.pending = m->d_pending;
     bp.down = m->d_down;
@@ -2038,7 +2144,9 @@ int write_tables(t2_batch *b)
             continue;
         std::vector<t2::ProblemDesc<T>> desc(g.idx.size());
         for (size_t j = 0; j < g.idx.size(); j++)
-            desc[j] = problem_desc<T>(b->members[g.idx[j]], b->ang_t[g.idx[j]]);
+            desc[j] = problem_desc<T>(
+                b->members[g.idx[j]], b->ang_t[g.idx[j]],
+                member_psi_out(b, b->members[g.idx[j]]));
         T2_HIP(hipMemcpy(g.d_desc, desc.data(),
                          sizeof(t2::ProblemDesc<T>) * desc.size(),
                          hipMemcpyHostToDevice));
@@ -2046,7 +2154,8 @@ int write_tables(t2_batch *b)
     if (b->d_problems) {
         std::vector<t2::BatchProblem<T>> probs(b->members.size());
         for (size_t i = 0; i < b->members.size(); i++)
-            probs[i] = batch_problem<T>(b->members[i], b->ang_t[i]);
+            probs[i] = batch_problem<T>(b->members[i], b->ang_t[i],
+                                        member_psi_out(b, b->members[i]));
         T2_HIP(hipMemcpy(b->d_problems, probs.data(),
                          sizeof(t2::BatchProblem<T>) * probs.size(),
                          hipMemcpyHostToDevice));
@@ -2054,13 +2163,16 @@ int write_tables(t2_batch *b)
     return T2_OK;
 }
 
-// Sets every member's angle-dependent source for the following sweeps.
-int batch_set_ang_t(t2_batch *b, const std::vector<const void *> &ang_t)
+// Sets every member's angle-dependent source, and whether the following
+// sweeps are correction sweeps (into the residual buffers).
+int batch_set_ang_t(t2_batch *b, const std::vector<const void *> &ang_t,
+                    bool corr = false)
 {
-    if (ang_t == b->ang_t)
+    if (ang_t == b->ang_t && corr == b->corr)
         return T2_OK;
     T2_HIP(hipStreamSynchronize(b->stream));
     b->ang_t = ang_t;
+    b->corr = corr;
     return b->precision == T2_FP64 ? write_tables<double>(b)
                                    : write_tables<float>(b);
 }
@@ -2119,7 +2231,8 @@ int build_level_group(t2_batch *b, LevelGroup &g)
             return fail(T2_ERR_HIP, std::string("batch merge: ") +
                                         hipGetErrorString(e));
         }
-        desc[j] = problem_desc<T>(m, b->ang_t[g.idx[j]]);
+        desc[j] = problem_desc<T>(m, b->ang_t[g.idx[j]],
+                                  member_psi_out(b, m));
     }
     (void)hipFree(d_off);
     T2_HIP(hipMalloc(reinterpret_cast<void **>(&g.d_active),
@@ -2189,7 +2302,7 @@ int build_batch_queue(t2_batch *b)
             T2_HIP(hipStreamSynchronize(m->stream));
             T2_TRY(build_queue(m, 0));
         }
-        probs[i] = batch_problem<T>(m, b->ang_t[i]);
+        probs[i] = batch_problem<T>(m, b->ang_t[i], member_psi_out(b, m));
     }
     // chunks of full size while a merged step gives every resident block
     // one; thinner steps are cut finer (down to one wavefront round of a
@@ -2243,11 +2356,15 @@ int batch_enqueue(t2_batch *b)
 {
     for (auto *m : b->members) {
         const int64_t n_lag = int64_t(m->sched.lagged.size());
-        if (n_lag > 0)
+        if (n_lag > 0 && !b->corr)
             T2_HIP(t2::launch_lag_gather<T>(
                 m->d_lag_key, m->d_lag_up, n_lag, m->nc, int(m->ng),
                 m->d_nbr, static_cast<const T *>(m->d_psi),
                 static_cast<T *>(m->d_lag_buf), b->stream));
+        else if (n_lag > 0)   // a correction's previous value is zero
+            T2_HIP(t2::launch_fill(static_cast<double *>(m->d_lag_buf), 0.0,
+                                   int64_t(sizeof(T)) * n_lag * 4 * m->ng / 8,
+                                   b->stream));
     }
     if (b->engine == T2_ENGINE_QUEUE) {
         t2::QueueState q{};
@@ -2392,7 +2509,8 @@ int batch_sweep(t2_batch *b)
             b->join_ev.push_back(ev);
         }
     }
-    if (!b->graph) {
+    hipGraphExec_t &exec = b->corr ? b->graph_corr : b->graph;
+    if (!exec) {
         hipGraph_t g = nullptr;
         T2_HIP(hipStreamBeginCapture(b->stream,
                                      hipStreamCaptureModeThreadLocal));
@@ -2405,11 +2523,15 @@ int batch_sweep(t2_batch *b)
             return rc;
         }
         T2_HIP(e);
-        e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         T2_HIP(e);
     }
-    T2_HIP(hipGraphLaunch(b->graph, b->stream));
+    T2_HIP(hipGraphLaunch(exec, b->stream));
+    if (!b->corr)   // the members' psi are replaced
+        for (size_t i = 0; i < b->members.size(); i++)
+            if (b->active[i])
+                b->members[i]->psi_corrected = false;
     return T2_OK;
 }
 
@@ -2737,7 +2859,12 @@ int t2_batch_solve_krylov(t2_batch *b, int64_t iter_max, double rtol,
     T2_TRY(batch_sweep_some(b, act));
     for (auto *m : b->members)
         T2_HIP(hipStreamSynchronize(m->stream));
-    return batch_check_abort(b);
+    T2_TRY(batch_check_abort(b));
+    for (size_t i = 0; i < nm; i++)
+        if (!gm[i].converged())
+            return noconv("batched GMRES, member " + std::to_string(i),
+                          gm[i].it, gm[i].rel_res(), rtol);
+    return T2_OK;
 }
 
 // The steps of the one-handle refined solve for every member, each stage in
@@ -2780,10 +2907,12 @@ static int batch_refined_body(t2_batch *b, int64_t iter_max,
                           V[i], nullptr));
     }
     T2_TRY(batch_gmres(b, gm, t0, nullptr));
+    std::vector<bool> conv1(nm);
     for (size_t i = 0; i < nm; i++) {
         t2_sweeper *m = b->members[i];
         st[i].iters_sweep = gm[i].it;
         st[i].residual_sweep = gm[i].rel_res();
+        conv1[i] = gm[i].converged();
         T2_TRY(total_source(m, m->d_qiso, m->d_phi));
     }
     T2_TRY(batch_sweep_some(b, all));
@@ -2828,10 +2957,26 @@ static int batch_refined_body(t2_batch *b, int64_t iter_max,
     T2_TRY(batch_gmres(b, gm, t0, done_at.data()));
     for (size_t i = 0; i < nm; i++) {
         t2_sweeper *m = b->members[i];
-        const int64_t n = m->n_phi();
         st[i].iters_correction = gm[i].it;
         st[i].residual_correction = gm[i].rel_res();
-        // y is D delta to the correction's tolerance: phi = phi1 + y
+        T2_TRY(total_source(m, m->d_qiso, m->d_phi));
+    }
+    // 4. psi32 swept again from step 1's phi, then every member's
+    // correction dpsi = L^-1 (r + M S D delta) over its residual in place
+    T2_TRY(batch_sweep_some(b, all));
+    for (size_t i = 0; i < nm; i++) {
+        T2_TRY(total_source(b->members[i], nullptr, y[i]));
+        res[i] = b->members[i]->d_res;
+    }
+    T2_TRY(batch_set_ang_t(b, res, true));
+    T2_TRY(batch_sweep_some(b, all));
+    T2_TRY(batch_set_ang_t(b, std::vector<const void *>(nm, nullptr)));
+    bool converged = true;
+    for (size_t i = 0; i < nm; i++) {
+        t2_sweeper *m = b->members[i];
+        const int64_t n = m->n_phi();
+        m->psi_corrected = true;
+        // phi = phi1 + y
         double corr = 0.0, base = 0.0;
         T2_TRY(dot(m, y[i], y[i], n, &corr));
         T2_HIP(t2::launch_axpby(1.0, m->d_phi_old, 1.0, y[i], n, m->stream));
@@ -2839,19 +2984,19 @@ static int batch_refined_body(t2_batch *b, int64_t iter_max,
                               hipMemcpyDeviceToDevice, m->stream));
         T2_TRY(dot(m, m->d_phi, m->d_phi, n, &base));
         st[i].correction_rel = base > 0 ? std::sqrt(corr / base) : 0.0;
-        T2_TRY(total_source(m, m->d_qiso, m->d_phi));
-    }
-    // 4. psi consistent with the final phi (FP32 storage)
-    T2_TRY(batch_sweep_some(b, all));
-    for (size_t i = 0; i < nm; i++) {
-        t2_sweeper *m = b->members[i];
         T2_HIP(hipStreamSynchronize(m->stream));
         st[i].sweeps = m->n_sweeps - sweeps0[i];
+        converged = converged && conv1[i] && gm[i].converged();
         if (info)
             info[i] = st[i];
         if (seconds)
             seconds[i] = done_at[i];
     }
+    if (!converged)
+        return fail(T2_ERR_NOCONV,
+                    "batched refined solve: a member's GMRES stage reached "
+                    "its iteration limit above its tolerance (see its "
+                    "residual_sweep / residual_correction)");
     return T2_OK;
 }
 
@@ -2873,8 +3018,13 @@ int t2_batch_solve_refined(t2_batch *b, int64_t iter_max, double rtol_sweep,
         const std::string msg = t2_last_error();
         (void)hipStreamSynchronize(b->stream);
         b->ang_t.assign(b->members.size(), nullptr);
+        b->corr = false;
         (void)(b->precision == T2_FP64 ? write_tables<double>(b)
                                        : write_tables<float>(b));
+        if (rc == T2_ERR_NOCONV) {   // results are whole; a timeout wins
+            const int a = batch_check_abort(b);
+            return a != T2_OK ? a : fail(rc, msg);
+        }
         return fail(rc, msg);
     }
     return batch_check_abort(b);
@@ -2935,7 +3085,13 @@ int t2_batch_solve_source_iteration(t2_batch *b, int64_t iter_max,
             }
         }
     }
-    return batch_check_abort(b);
+    T2_TRY(batch_check_abort(b));
+    for (size_t i = 0; i < nm; i++)
+        if (err[i] > err_max)
+            return noconv("batched source iteration, member " +
+                              std::to_string(i),
+                          it[i], err[i], err_max);
+    return T2_OK;
 }
 
 int t2_batch_info(const t2_batch *b, int64_t *n_steps, int64_t *n_items)

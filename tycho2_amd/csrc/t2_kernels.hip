@@ -1093,6 +1093,7 @@ __global__ void ca_to_ac_kernel(const double *__restrict__ in,
 
 template <typename T>
 __global__ void ac_to_ca_kernel(const T *__restrict__ in,
+                                const T *__restrict__ corr,
                                 double *__restrict__ out, int64_t nc,
                                 int64_t na, int64_t vg)
 {
@@ -1102,7 +1103,11 @@ __global__ void ac_to_ca_kernel(const T *__restrict__ in,
         const int64_t e = i % vg;
         const int64_t ca = i / vg;   // output index (c, a)
         const int64_t a = ca % na, c = ca / na;
-        out[i] = double(in[(a * nc + c) * vg + e]);
+        const int64_t j = (a * nc + c) * vg + e;
+        double v = double(in[j]);
+        if (corr)   // the correction is added in FP64
+            v += double(corr[j]);
+        out[i] = v;
     }
 }
 
@@ -1504,10 +1509,10 @@ hipError_t launch_ca_to_ac(const double *in, T *out, int64_t nc,
 
 template <typename T>
 hipError_t launch_ac_to_ca(const T *in, double *out, int64_t nc, int64_t na,
-                           int64_t vg, hipStream_t stream)
+                           int64_t vg, hipStream_t stream, const T *corr)
 {
     hipLaunchKernelGGL((ac_to_ca_kernel<T>), dim3(stream_grid(nc * na * vg)),
-                       dim3(256), 0, stream, in, out, nc, na, vg);
+                       dim3(256), 0, stream, in, corr, out, nc, na, vg);
     return hipGetLastError();
 }
 
@@ -1603,7 +1608,8 @@ hipError_t launch_fill(double *x, double v, int64_t n, hipStream_t stream)
                                                const double *, int64_t, int,  \
                                                T *, hipStream_t);             \
     template hipError_t launch_ac_to_ca<T>(const T *, double *, int64_t,      \
-                                           int64_t, int64_t, hipStream_t);    \
+                                           int64_t, int64_t, hipStream_t,     \
+                                           const T *);                        \
     template hipError_t launch_ca_to_ac<T>(const double *, T *, int64_t,      \
                                            int64_t, int64_t, hipStream_t);
 

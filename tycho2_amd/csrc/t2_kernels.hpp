@@ -224,9 +224,11 @@ hipError_t launch_total_source(const double *q, const double *sigma_s,
 template <typename T>
 hipError_t launch_ca_to_ac(const double *in, T *out, int64_t nc,
                            int64_t na, int64_t vg, hipStream_t stream);
+// out = in (+ corr, when not null, added in FP64)
 template <typename T>
 hipError_t launch_ac_to_ca(const T *in, double *out, int64_t nc, int64_t na,
-                           int64_t vg, hipStream_t stream);
+                           int64_t vg, hipStream_t stream,
+                           const T *corr = nullptr);
 
 // Deterministic reductions: partial sums per block into `partial`
 // (kRedBlocks entries per result), then one block adds them in order.

@@ -58,6 +58,18 @@ void check(int rc)
         throw std::runtime_error(std::string("engine: ") + t2_last_error());
 }
 
+// A solve that stopped at iterMax keeps its last iterate, as the reference's
+// loops do (src/SourceIteration.cc:144-176); the engine reports it, and the
+// run goes on with a warning.
+void check_solve(int rc)
+{
+    if (rc == T2_ERR_NOCONV) {
+        std::fprintf(stderr, "sweep_hip.x: warning: %s\n", t2_last_error());
+        return;
+    }
+    check(rc);
+}
+
 // ------------------------------------------------------------------ deck
 // key value pairs, '#' comments (the reference's KeyValueReader format)
 struct Deck {
@@ -89,6 +101,10 @@ struct Deck {
         return it->second;
     }
     double num(const std::string &k) const { return std::stod(str(k)); }
+    double num(const std::string &k, double def) const
+    {
+        return kv.count(k) ? std::stod(kv.at(k)) : def;
+    }
     bool flag(const std::string &k, const std::string &def = "false") const
     {
         return str(k, def) == "true";
@@ -396,6 +412,10 @@ public:
     {
         const int prec = deck.str("Precision", "FP64") == "FP32" ? T2_FP32
                                                                  : T2_FP64;
+        if (prec == T2_FP32 && !deck.flag("DeviceLoop"))
+            throw std::runtime_error(
+                "Precision FP32 runs the refined solve on the device; set "
+                "DeviceLoop true");
         // "SweepEngine Queue" selects the persistent ready-queue launch
         // (default: per-step launches replayed from a hipGraph)
         const std::string engine = deck.str("SweepEngine", "Levels");
@@ -506,13 +526,30 @@ void SweeperHIP::solve()
         return;
     }
     check(t2_set_source_angular(c_engine, source.v.data()));
-    if (c_deck.flag("SourceIteration", "true"))
-        check(t2_solve_source_iteration(c_engine, int64_t(c_deck.num("iterMax")),
-                                        c_deck.num("errMax"), &iterations,
-                                        nullptr));
-    else
-        check(t2_solve_krylov(c_engine, int64_t(c_deck.num("iterMax")),
-                              c_deck.num("errMax"), 30, &iterations, nullptr));
+    const int64_t iter_max = int64_t(c_deck.num("iterMax"));
+    if (c_deck.str("Precision", "FP64") == "FP32") {
+        // FP32 sweeps inside FP64 defect correction: GMRES stages to the
+        // deck's RefineTolSweep / RefineTolCorrection (defaults 1e-6 / 1e-5,
+        // phi and psi to ~1e-12 of the FP64 solution); psi comes back as
+        // psi32 plus its correction, in FP64
+        t2_refine_info info{};
+        check_solve(t2_solve_refined(
+            c_engine, iter_max, c_deck.num("RefineTolSweep", 1e-6),
+            c_deck.num("RefineTolCorrection", 1e-5), 30, &info, nullptr));
+        iterations = info.iters_sweep + info.iters_correction;
+        std::printf("   refined solve: %lld + %lld GMRES iterations, "
+                    "%lld FP32 sweeps, correction %e\n",
+                    (long long)info.iters_sweep,
+                    (long long)info.iters_correction, (long long)info.sweeps,
+                    info.correction_rel);
+    } else if (c_deck.flag("SourceIteration", "true")) {
+        check_solve(t2_solve_source_iteration(c_engine, iter_max,
+                                              c_deck.num("errMax"),
+                                              &iterations, nullptr));
+    } else {
+        check_solve(t2_solve_krylov(c_engine, iter_max, c_deck.num("errMax"),
+                                    30, &iterations, nullptr));
+    }
     check(t2_get_psi(c_engine, psi.v.data()));
 }
 

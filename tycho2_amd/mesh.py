@@ -308,7 +308,18 @@ def kuhn_cube(n: int, jitter: float = 0.0, seed: int = 0,
     # faces: face k of a cell is opposite vertex k
     opp = np.array([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]])
     slots = np.sort(cell_nodes[:, opp], axis=2).reshape(-1, 3)  # (4nc, 3)
-    face_nodes, inverse = np.unique(slots, axis=0, return_inverse=True)
+    bits = max(1, int(coords.shape[0] - 1).bit_length())
+    if 3 * bits <= 63:
+        # one int64 key per sorted node triple, ordered as the triples are
+        # lexicographically (the same face numbering as a row-wise unique)
+        keys = (slots[:, 0] << (2 * bits)) | (slots[:, 1] << bits) | \
+            slots[:, 2]
+        ukeys, inverse = np.unique(keys, return_inverse=True)
+        mask = (1 << bits) - 1
+        face_nodes = np.stack([ukeys >> (2 * bits), (ukeys >> bits) & mask,
+                               ukeys & mask], axis=1)
+    else:
+        face_nodes, inverse = np.unique(slots, axis=0, return_inverse=True)
     inverse = inverse.ravel()
     nf = face_nodes.shape[0]
     cell_faces = inverse.reshape(nc, 4).astype(np.int64)

@@ -115,21 +115,42 @@ def scattering(n: int = 50, sn: int = 12, ng: int = 8) -> Problem:
 
 
 def batch(count: int = 48, seed: int = BATCH_SEED, n_range=(10, 45),
-          angle_orders=(4, 6, 8, 12), groups=(1, 4, 16, 32)):
-    """Seeded list of independent problems of uneven size."""
+          angle_orders=(4, 6, 8, 12), groups=(1, 4, 16, 32), only=None):
+    """Seeded list of independent problems of uneven size.  ``only`` (a set
+    of indices) builds just those members, with the same draws."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
         n = int(rng.integers(n_range[0], n_range[1] + 1))
         sn = int(rng.choice(angle_orders))
         ng = int(rng.choice(groups))
+        nc = 6 * n ** 3
+        st = np.exp(rng.uniform(np.log(0.5), np.log(5.0), nc))
+        ratio = rng.uniform(0.3, 0.95, nc)
+        if only is not None and i not in only:
+            continue
         sm = meshmod.kuhn_cube(n, 0.15, seed=JITTER_SEED + i)
-        st = np.exp(rng.uniform(np.log(0.5), np.log(5.0), sm.n_cells))
-        ratio = rng.uniform(0.3, 0.95, sm.n_cells)
+        assert sm.n_cells == nc
         out.append(_make(f"batch{i}", sm, sn, ng, st[:, None],
                          (ratio * st)[:, None], _uniform_q(sm.n_cells, ng),
                          dict(method="krylov", iter_max=300, tol=1e-10,
                               restart=20)))
+    return out
+
+
+def batch_sizes(count: int = 48, seed: int = BATCH_SEED, n_range=(10, 45),
+                angle_orders=(4, 6, 8, 12), groups=(1, 4, 16, 32)):
+    """(hexes per side, angles, groups) of every member of ``batch`` without
+    building a mesh."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        sn = int(rng.choice(angle_orders))
+        ng = int(rng.choice(groups))
+        rng.uniform(size=6 * n ** 3)     # sigma_t draws
+        rng.uniform(size=6 * n ** 3)     # sigma_s / sigma_t draws
+        out.append((n, 2 * sn * sn, ng))
     return out
 
 

@@ -95,6 +95,7 @@ SIGNATURES = {
     "t2_set_phi": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "t2_get_psi_angle": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64,
                                         _dp]),
+    "t2_set_alloc_fail": (ctypes.c_int, [ctypes.c_int64]),
     "t2_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "t2_get_stream": (ctypes.c_int, [ctypes.c_void_p,
                                      ctypes.POINTER(ctypes.c_void_p)]),
@@ -158,6 +159,16 @@ class T2Error(RuntimeError):
         self.code = code
 
 
+ERR_NOCONV = -6
+
+
+class NoConvergence(T2Error):
+    """T2_ERR_NOCONV: a solve stopped at its iteration limit above its
+    tolerance.  The handle holds the last iterate; ``result`` is what the
+    call returns on success (iteration history, counts, infos)."""
+    result = None
+
+
 def lib():
     """Load the engine library (raises if it has not been built)."""
     global _LIB
@@ -183,7 +194,18 @@ def library_path() -> str:
 def _check(rc: int) -> None:
     if rc != 0:
         msg = lib().t2_last_error()
-        raise T2Error(rc, msg.decode() if msg else "")
+        cls = NoConvergence if rc == ERR_NOCONV else T2Error
+        raise cls(rc, msg.decode() if msg else "")
+
+
+def _solved(rc: int, result):
+    """`result` of a solve call, or NoConvergence carrying it."""
+    try:
+        _check(rc)
+    except NoConvergence as e:
+        e.result = result
+        raise
+    return result
 
 
 def _f64(a):
@@ -196,6 +218,12 @@ def _i64(a):
 
 def _ptr(a, kind=_dp):
     return a.ctypes.data_as(kind)
+
+
+def set_alloc_fail(nth: int) -> None:
+    """Testing hook: the nth device allocation from now fails with
+    T2_ERR_NOMEM (0 disables)."""
+    _check(lib().t2_set_alloc_fail(int(nth)))
 
 
 def mesh_geometry(smesh, conn):
@@ -377,18 +405,18 @@ class SweeperHIP:
     def solve_source_iteration(self, iter_max=100, err_max=1e-10):
         hist = np.zeros(max(int(iter_max), 1))
         nit = ctypes.c_int64(0)
-        _check(lib().t2_solve_source_iteration(
+        rc = lib().t2_solve_source_iteration(
             self.handle, int(iter_max), float(err_max), ctypes.byref(nit),
-            _ptr(hist)))
-        return hist[:nit.value].copy()
+            _ptr(hist))
+        return _solved(rc, hist[:nit.value].copy())
 
     def solve_krylov(self, iter_max=100, rtol=1e-10, restart=30):
         hist = np.zeros(max(int(iter_max), 1))
         nit = ctypes.c_int64(0)
-        _check(lib().t2_solve_krylov(self.handle, int(iter_max), float(rtol),
-                                     int(restart), ctypes.byref(nit),
-                                     _ptr(hist)))
-        return hist[:nit.value].copy()
+        rc = lib().t2_solve_krylov(self.handle, int(iter_max), float(rtol),
+                                   int(restart), ctypes.byref(nit),
+                                   _ptr(hist))
+        return _solved(rc, hist[:nit.value].copy())
 
     def solve_refined(self, iter_max=300, rtol_sweep=1e-6,
                       rtol_correction=1e-5, restart=20, want_phi1=False):
@@ -396,11 +424,12 @@ class SweeperHIP:
         (info dict, phi after the FP32 stage or None)."""
         info = T2RefineInfo()
         phi1 = np.empty((self.nc, 4, self.ng)) if want_phi1 else None
-        _check(lib().t2_solve_refined(
+        rc = lib().t2_solve_refined(
             self.handle, int(iter_max), float(rtol_sweep),
             float(rtol_correction), int(restart), ctypes.byref(info),
-            _ptr(phi1) if want_phi1 else None))
-        return {k: getattr(info, k) for k, _ in T2RefineInfo._fields_}, phi1
+            _ptr(phi1) if want_phi1 else None)
+        return _solved(rc, ({k: getattr(info, k)
+                             for k, _ in T2RefineInfo._fields_}, phi1))
 
     def krylov_apply(self, x: np.ndarray) -> np.ndarray:
         """b = (I - D L^-1 M S) x for phi-shaped x [c][v][g] (the
@@ -548,20 +577,20 @@ class SweepBatch:
         n = len(self.members)
         its = np.zeros(n, np.int64)
         sec = np.zeros(n)
-        _check(lib().t2_batch_solve_krylov(self.handle, int(iter_max),
-                                           float(rtol), int(restart),
-                                           _ptr(its, _ip), _ptr(sec)))
-        return its, sec
+        rc = lib().t2_batch_solve_krylov(self.handle, int(iter_max),
+                                         float(rtol), int(restart),
+                                         _ptr(its, _ip), _ptr(sec))
+        return _solved(rc, (its, sec))
 
     def solve_source_iteration(self, iter_max: int = 1000,
                                err_max: float = 1e-10):
         n = len(self.members)
         its = np.zeros(n, np.int64)
         sec = np.zeros(n)
-        _check(lib().t2_batch_solve_source_iteration(
+        rc = lib().t2_batch_solve_source_iteration(
             self.handle, int(iter_max), float(err_max), _ptr(its, _ip),
-            _ptr(sec)))
-        return its, sec
+            _ptr(sec))
+        return _solved(rc, (its, sec))
 
     def solve_refined(self, iter_max: int = 300, rtol_sweep: float = 1e-6,
                       rtol_correction: float = 1e-5, restart: int = 20):
@@ -571,11 +600,12 @@ class SweepBatch:
         n = len(self.members)
         infos = (T2RefineInfo * n)()
         sec = np.zeros(n)
-        _check(lib().t2_batch_solve_refined(
+        rc = lib().t2_batch_solve_refined(
             self.handle, int(iter_max), float(rtol_sweep),
-            float(rtol_correction), int(restart), infos, _ptr(sec)))
-        return [{k: getattr(x, k) for k, _ in T2RefineInfo._fields_}
-                for x in infos], sec
+            float(rtol_correction), int(restart), infos, _ptr(sec))
+        return _solved(rc, ([{k: getattr(x, k)
+                              for k, _ in T2RefineInfo._fields_}
+                             for x in infos], sec))
 
     def info(self) -> dict:
         n_steps, n_items = ctypes.c_int64(), ctypes.c_int64()
