@@ -108,6 +108,14 @@ int t2_sweep_levels(int64_t n_cells, const int64_t *adj_cell,
                     const double *area, const double *normal,
                     int64_t n_angles, const double *omega, int32_t *levels,
                     int64_t *n_lagged);
+/* The same with the dependency rule of t2_set_dependency_tolerance:
+ * dep_tol >= 0 makes only faces with Omega.n < -dep_tol dependencies and
+ * lags the other interior faces with Omega.n <= 0 from the start (counted
+ * in *n_lagged); dep_tol < 0 is t2_sweep_levels. */
+int t2_sweep_levels_tol(int64_t n_cells, const int64_t *adj_cell,
+                        const double *area, const double *normal,
+                        int64_t n_angles, const double *omega,
+                        double dep_tol, int32_t *levels, int64_t *n_lagged);
 
 /* Traversal priorities of the reference's graph traverser on one rank,
  * priorities[a][c] (Priorities::calcPriorities, src/Priorities.cc:396-435;
@@ -187,6 +195,20 @@ int t2_set_schedule(t2_sweeper *s, int angle_batches);
  * results do not depend on it.  No reference counterpart (the reference's
  * angle groups are OpenMP threads, src/GraphTraverser.cc:151-176). */
 int t2_set_concurrent_batches(t2_sweeper *s, int concurrent);
+
+/* Dependency rule of the sweep schedule.  tol < 0 (default): the graph
+ * traverser's, every interior face with Omega.n <= 0 waits for its upwind
+ * cell (src/TychoMesh.hh:107-110, src/GraphTraverser.cc:433-457).
+ * tol >= 0: the precomputed schedule's of SweepType OriginalTycho1/2
+ * (TOL = 1e-10, src/SweepSchedule.cc:57,108-125): only faces with
+ * Omega.n < -tol are dependencies; an interior face with
+ * -tol <= Omega.n < 0 reads its upwind cell's psi from before the sweep, as
+ * a lagged face does (the reference's Sweeper reads whatever its psi array
+ * holds when the cell is solved, src/Sweeper.cc:281-300, which is that value
+ * or this sweep's depending on the order; here it is order independent).
+ * Faces with Omega.n = 0 couple nothing either way (src/Transport.cc:113,
+ * 184).  Cycles that remain are broken as in t2_sweep_levels. */
+int t2_set_dependency_tolerance(t2_sweeper *s, double tol);
 
 /* Order of the items within a step (any order is valid; results do not
  * depend on it): T2_ORDER_CELL by angle then cell; T2_ORDER_FACES (default)

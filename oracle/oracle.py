@@ -129,6 +129,22 @@ def _sig(sig, nc, ng):
     return _f64(s)
 
 
+class dependency_tolerance:
+    """Context manager: the oracle's sweeps inside use the precomputed
+    schedule's dependency rule with this tolerance (faces with
+    -tol <= Omega.n <= 0 lagged from the start; SweepSchedule.cc:57)."""
+
+    def __init__(self, tol: float):
+        self.tol = float(tol)
+
+    def __enter__(self):
+        lib().or_set_dependency_tolerance(ctypes.c_double(self.tol))
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_dependency_tolerance(ctypes.c_double(-1.0))
+
+
 def sweep(mesh: Mesh, omega, ng, sigma_t, source, psi=None, method=NOPIVOT,
           nthreads=0, angles=None):
     """psi[c][a][v][g] = L^-1 source.  Returns (psi, n_lagged)."""

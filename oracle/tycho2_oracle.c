@@ -452,6 +452,30 @@ static void update_cell(const or_mesh *m, int64_t c, int64_t a, int64_t na,
  * Pass 1 fixes the order and the lagged faces, the snapshot of the lagged
  * faces' upwind values is taken, and pass 2 computes in that order.
  */
+/* Dependency rule: < 0 the graph traverser's (every interior face with
+ * Omega.n <= 0 is a dependency, TychoMesh.hh:107-110); >= 0 the precomputed
+ * schedule's of SweepType OriginalTycho1/2 (SweepSchedule.cc:57,108-125:
+ * only Omega.n < -tol is a dependency), the other interior faces with
+ * Omega.n <= 0 lagged from the start. */
+static double g_dep_tol = -1.0;
+
+void or_set_dependency_tolerance(double tol) { g_dep_tol = tol; }
+
+static void add_lag(or_lag **lag, int64_t *n_lag, int64_t *cap, int64_t c,
+                    int k)
+{
+    if (*n_lag == *cap) {
+        *cap = *cap ? 2 * *cap : 16;
+        *lag = realloc(*lag, sizeof(or_lag) * *cap);
+        if (!*lag)
+            abort();
+    }
+    (*lag)[*n_lag].cell = c;
+    (*lag)[*n_lag].face = k;
+    (*lag)[*n_lag].vals = NULL;
+    (*n_lag)++;
+}
+
 static int64_t sweep_angle(const or_mesh *m, int64_t a, int64_t na,
                            const double *om, int64_t ng,
                            const double *sigma_t, const double *source,
@@ -462,14 +486,18 @@ static int64_t sweep_angle(const or_mesh *m, int64_t a, int64_t na,
     const int64_t nc = m->nc;
     int64_t head = 0, tail = 0, n_lag = 0, cap = 0;
     or_lag *lag = NULL;
+    const double tol = g_dep_tol;
     /* met[c][k]: 0 unmet incoming interior face, 1 released by the upwind
      * cell, 2 lagged, 3 not a dependency */
     for (int64_t c = 0; c < nc; c++) {
         int64_t d = 0;
         for (int k = 0; k < 4; k++) {
-            int dep = m->adj[4 * c + k] >= 0 &&
-                      !(omega_dot_n(om, m->normal + (4 * c + k) * 3) > 0);
-            met[4 * c + k] = dep ? 0 : 3;
+            double dn = omega_dot_n(om, m->normal + (4 * c + k) * 3);
+            int in = m->adj[4 * c + k] >= 0 && !(dn > 0);
+            int dep = in && (tol < 0 || dn < -tol);
+            met[4 * c + k] = dep ? 0 : in ? 2 : 3;
+            if (in && !dep)
+                add_lag(&lag, &n_lag, &cap, c, k);
             d += dep;
         }
         indeg[c] = d;
@@ -502,14 +530,7 @@ static int64_t sweep_angle(const or_mesh *m, int64_t a, int64_t na,
             }
             if (bc < 0)
                 abort(); /* unreachable: a stalled cell has an unmet face */
-            if (n_lag == cap) {
-                cap = cap ? 2 * cap : 16;
-                lag = realloc(lag, sizeof(or_lag) * cap);
-            }
-            lag[n_lag].cell = bc;
-            lag[n_lag].face = bk;
-            lag[n_lag].vals = NULL;
-            n_lag++;
+            add_lag(&lag, &n_lag, &cap, bc, bk);
             met[4 * bc + bk] = 2;
             if (--indeg[bc] == 0) {
                 queue[tail++] = bc;

@@ -71,6 +71,11 @@ int or_sweep(const or_mesh *m, int64_t na, const double *omega, int64_t ng,
              int method, int nthreads, int64_t *n_lagged);
 
 /* Sweep a subset of angles [a0, a1) only (CPU-baseline sampling). */
+/* Dependency rule of every following sweep: tol < 0 (default) the graph
+ * traverser's, tol >= 0 the precomputed schedule's (faces with
+ * -tol <= Omega.n <= 0 lagged from the start).  Process-wide. */
+void or_set_dependency_tolerance(double tol);
+
 int or_sweep_angles(const or_mesh *m, int64_t na, const double *omega,
                     int64_t ng, const double *sigma_t, const double *source,
                     double *psi, int method, int nthreads, int64_t a0,

@@ -69,12 +69,19 @@ def test_driver_usage_and_errors(tmp_path):
     ("DeviceLoop false", "CramerIntel"),
     ("DeviceLoop false\nSweepSchedule Priorities", "NoPivot"),
     ("DeviceLoop true\nSweepEngine Queue\nSweepSchedule Priorities",
-     "NoPivot")])
+     "NoPivot"),
+    ("DeviceLoop false\nSWEEPTYPE OriginalTycho1", "NoPivot"),
+    ("DeviceLoop true\nSWEEPTYPE OriginalTycho2", "NoPivot"),
+    ("DeviceLoop true\nSWEEPTYPE TraverseGraph", "NoPivot")])
 def test_driver_reproduces_gold(tmp_path, gold, extra, gauss):
     print(f"driver gold {extra} GaussElim {gauss}", flush=True)
     out = str(tmp_path / "out.psi")
+    sweep_type = "HIP"
+    if "SWEEPTYPE" in extra:   # the reference's other single-rank sweepers
+        extra, sweep_type = extra.split("\nSWEEPTYPE ")
     deck = GOLD_DECK.format(out=out, extra=extra).replace(
-        "GaussElim       NoPivot", f"GaussElim       {gauss}")
+        "GaussElim       NoPivot", f"GaussElim       {gauss}").replace(
+        "SweepType       HIP", f"SweepType       {sweep_type}")
     r = run(deck, tmp_path)
     assert r.returncode == 0, r.stderr
     psi = psi_io.read_psi(out)

@@ -603,3 +603,47 @@ def test_refined_on_concurrent_paths_bitwise():
         assert np.array_equal(m.phi(), phi)
     for s in own + conc + mem:
         s.close()
+
+
+@pytest.mark.parametrize("engine", [0, 1])
+def test_dependency_tolerance_matches_oracle(engine):
+    """The precomputed schedule's rule (faces with |Omega.n| <= 1e-10 are
+    not dependencies and read the pre-sweep psi; SweepType
+    OriginalTycho1/2) on an unjittered Kuhn cube with S2 directions along
+    its diagonal faces: the lagged faces, one sweep and a source iteration
+    equal the oracle's under the same rule, on the level engine and the
+    ready queue."""
+    from tycho2_amd.sweeper import ENGINE_QUEUE
+    print(f"dependency tolerance engine={engine}", flush=True)
+    sm = meshmod.kuhn_cube(3, 0.0)
+    conn = meshmod.tycho_connectivity(sm)
+    omega, wt = orc.quadrature(2)
+    ng = 4
+    rng = np.random.default_rng(13)
+    st = rng.uniform(1.0, 2.0, (sm.n_cells, ng))
+    ss = 0.5 * st
+    src = rng.uniform(0.1, 1.0, (sm.n_cells, omega.shape[0], 4, ng))
+    prev = rng.uniform(0.0, 1.0, src.shape)
+    m = orc.build_mesh(sm)
+    with orc.dependency_tolerance(1e-10):
+        ref, lag = orc.sweep(m, omega, ng, st, src, psi=prev.copy())
+        q1 = orc.isotropic_source(sm.n_cells, omega.shape[0], ng,
+                                  np.ones((sm.n_cells, ng)))
+        _, rphi, rhist = orc.source_iteration(m, omega, wt, ng, st, ss, q1,
+                                              500, 1e-13)
+    with SweeperHIP(sm, conn, omega, wt, ng, st, ss, FP64) as s:
+        n0 = s.info()["n_lagged"]
+        s.set_dependency_tolerance(1e-10)
+        if engine:
+            s.set_engine(ENGINE_QUEUE)
+        assert s.info()["n_lagged"] == lag > n0
+        psi = prev.copy()
+        s.sweep(psi, src)
+        assert maxrel(psi, ref) <= 1e-12
+        s.set_source_isotropic(np.ones((sm.n_cells, 4, ng)))
+        hist = s.solve_source_iteration(500, 1e-13)
+        assert len(hist) == len(rhist)
+        assert maxrel(s.phi(), rphi) <= 1e-12
+        # back to the traverser's rule
+        s.set_dependency_tolerance(-1)
+        assert s.info()["n_lagged"] == n0

@@ -54,6 +54,7 @@ struct t2_sweeper {
     int ipl = 1;                     // items per lane (level engine)
     int intra = -1, inter = 0;       // priority schedule (intra < 0: levels)
     int64_t step_width = 0;
+    double dep_tol = t2::kTraverseRule;   // dependency rule of the schedule
     bool use_graph = true;
     bool concurrent = false;         // angle batches on parallel streams
     std::vector<hipStream_t> side;   // their extra streams
@@ -789,10 +790,10 @@ int build_schedule_device(t2_sweeper *s)
     if (s->intra >= 0)
         T2_TRY(t2::build_priority_schedule(s->hm, na, s->omega_h.data(),
                                            s->intra, s->inter, s->step_width,
-                                           s->order, s->sched));
+                                           s->order, s->sched, s->dep_tol));
     else
         T2_TRY(t2::build_schedule(s->hm, na, s->omega_h.data(), s->batches,
-                                  s->order, s->sched));
+                                  s->order, s->sched, s->dep_tol));
     const size_t n_items = size_t(nc) * na;
     std::vector<double> geo(size_t(nc) * t2::kGeoStride, 0.0);
     std::vector<int4> adj(nc);
@@ -1366,6 +1367,20 @@ int t2_set_order(t2_sweeper *s, int order)
         return T2_OK;
     T2_HIP(hipStreamSynchronize(s->stream));
     s->order = order;
+    return build_schedule_device(s);
+}
+
+int t2_set_dependency_tolerance(t2_sweeper *s, double tol)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    if (std::isnan(tol))
+        return fail(T2_ERR_ARG, "dependency tolerance is NaN");
+    const double rule = tol < 0 ? t2::kTraverseRule : tol;
+    if (rule == s->dep_tol)
+        return T2_OK;
+    T2_HIP(hipStreamSynchronize(s->stream));
+    s->dep_tol = rule;
     return build_schedule_device(s);
 }
 

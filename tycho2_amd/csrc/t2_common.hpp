@@ -61,14 +61,23 @@ struct Schedule {
     std::vector<int64_t> batch_steps;  // angle batch b: steps [b], [b + 1]
 };
 
+// Dependency rule of the level sets.  dep_tol < 0: the graph traverser's,
+// every interior face with Omega.n <= 0 is a dependency
+// (src/TychoMesh.hh:107-110).  dep_tol >= 0: the precomputed schedule's,
+// only faces with Omega.n < -dep_tol are (src/SweepSchedule.cc:57,108-125);
+// the other interior faces with Omega.n <= 0 are lagged from the start.
+constexpr double kTraverseRule = -1.0;
+
 // Levels of one angle; returns the lagged faces appended to `lagged`.
 void angle_levels(const HostMesh &m, const double *om, int32_t angle,
                   int32_t *level, std::vector<LaggedFace> &lagged,
                   std::vector<int32_t> &scratch_indeg,
-                  std::vector<int32_t> &scratch_frontier);
+                  std::vector<int32_t> &scratch_frontier,
+                  double dep_tol = kTraverseRule);
 
 int build_levels(const HostMesh &m, int64_t na, const double *omega,
-                 int32_t *levels, std::vector<LaggedFace> &lagged);
+                 int32_t *levels, std::vector<LaggedFace> &lagged,
+                 double dep_tol = kTraverseRule);
 
 // Angles swept together: octant by octant (sign pattern of Omega), then by
 // index, cut into `batches` nearly equal consecutive groups.
@@ -79,7 +88,8 @@ std::vector<int32_t> angle_batches(int64_t na, const double *omega,
 // angle and, for order T2_ORDER_FACES, the pattern of incoming interior
 // faces, then by cell.
 int build_schedule(const HostMesh &m, int64_t na, const double *omega,
-                   int batches, int order, Schedule &s);
+                   int batches, int order, Schedule &s,
+                   double dep_tol = kTraverseRule);
 
 // Traversal priorities of the reference's graph traverser on one rank
 // (src/Priorities.cc:396-435): intra-angle rule 0 random, 1 b-level,
@@ -100,6 +110,7 @@ int sweep_priorities(const HostMesh &m, int64_t na, const double *omega,
 // their pattern of incoming interior faces (T2_ORDER_FACES).
 int build_priority_schedule(const HostMesh &m, int64_t na,
                             const double *omega, int intra, int inter,
-                            int64_t width, int order, Schedule &s);
+                            int64_t width, int order, Schedule &s,
+                            double dep_tol = kTraverseRule);
 
 }  // namespace t2

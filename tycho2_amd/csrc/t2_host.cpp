@@ -156,9 +156,16 @@ int build_host_mesh(int64_t nc, int64_t nn, const double *coords,
 // the angular flux its upwind cell held before the sweep.  The set of
 // finished cells at a stall is the same for every traversal order, so the
 // lagged set is the one the oracle's FIFO traversal finds.
+//
+// With dep_tol >= 0 (the precomputed schedule's rule) only faces with
+// Omega.n < -dep_tol are dependencies; an interior face with
+// -dep_tol <= Omega.n <= 0 is lagged from the start (its upwind value is the
+// pre-sweep one whatever the order; the reference's Sweeper reads whatever
+// its psi array holds at that moment, src/Sweeper.cc:281-300).
 void angle_levels(const HostMesh &m, const double *om, int32_t angle,
                   int32_t *level, std::vector<LaggedFace> &lagged,
-                  std::vector<int32_t> &indeg, std::vector<int32_t> &frontier)
+                  std::vector<int32_t> &indeg, std::vector<int32_t> &frontier,
+                  double dep_tol)
 {
     const int64_t nc = m.nc;
     // met[c][k]: 0 unmet dependency, 1 released, 2 lagged, 3 no dependency
@@ -168,9 +175,13 @@ void angle_levels(const HostMesh &m, const double *om, int32_t angle,
     for (int64_t c = 0; c < nc; c++) {
         int d = 0;
         for (int k = 0; k < 4; k++) {
-            const bool dep = m.adj[4 * c + k] >= 0 &&
-                             !(omega_dot_n(om, &m.normal[(4 * c + k) * 3]) > 0);
-            met[4 * c + k] = dep ? 0 : 3;
+            const int64_t u = m.adj[4 * c + k];
+            const double dn = omega_dot_n(om, &m.normal[(4 * c + k) * 3]);
+            const bool in = u >= 0 && !(dn > 0);
+            const bool dep = in && (dep_tol < 0 || dn < -dep_tol);
+            met[4 * c + k] = dep ? 0 : in ? 2 : 3;
+            if (in && !dep)
+                lagged.push_back({angle, int32_t(c), k, int32_t(u)});
             d += dep;
         }
         indeg[c] = d;
@@ -247,7 +258,8 @@ static int n_workers(int64_t jobs)
 }
 
 int build_levels(const HostMesh &m, int64_t na, const double *omega,
-                 int32_t *levels, std::vector<LaggedFace> &lagged)
+                 int32_t *levels, std::vector<LaggedFace> &lagged,
+                 double dep_tol)
 {
     const int nw = n_workers(na);
     std::vector<std::vector<LaggedFace>> lag(nw);
@@ -256,7 +268,7 @@ int build_levels(const HostMesh &m, int64_t na, const double *omega,
         std::vector<int32_t> indeg, frontier;
         for (int64_t a; (a = next.fetch_add(1)) < na;)
             angle_levels(m, omega + 3 * a, int32_t(a), levels + a * m.nc,
-                         lag[w], indeg, frontier);
+                         lag[w], indeg, frontier, dep_tol);
     };
     std::vector<std::thread> pool;
     for (int w = 1; w < nw; w++)
@@ -300,7 +312,7 @@ std::vector<int32_t> angle_batches(int64_t na, const double *omega,
 // L_b the most levels of any of its angles; step off_b + l holds every
 // (angle, cell) of the batch at level l, angle-major and cell-ascending.
 int build_schedule(const HostMesh &m, int64_t na, const double *omega,
-                   int batches, int order, Schedule &s)
+                   int batches, int order, Schedule &s, double dep_tol)
 {
     const int64_t nc = m.nc;
     int cb = 1;
@@ -319,7 +331,7 @@ int build_schedule(const HostMesh &m, int64_t na, const double *omega,
         return fail(T2_ERR_NOMEM, "host memory for level sets");
     }
     s.lagged.clear();
-    int rc = build_levels(m, na, omega, levels.data(), s.lagged);
+    int rc = build_levels(m, na, omega, levels.data(), s.lagged, dep_tol);
     if (rc)
         return rc;
     const std::vector<int32_t> batch = angle_batches(na, omega, batches);
@@ -583,7 +595,8 @@ int sweep_priorities(const HostMesh &m, int64_t na, const double *omega,
 
 int build_priority_schedule(const HostMesh &m, int64_t na,
                             const double *omega, int intra, int inter,
-                            int64_t width, int order, Schedule &s)
+                            int64_t width, int order, Schedule &s,
+                            double dep_tol)
 {
     const int64_t nc = m.nc;
     int cb = 1;
@@ -604,7 +617,7 @@ int build_priority_schedule(const HostMesh &m, int64_t na,
         return fail(T2_ERR_NOMEM, "host memory for level sets");
     }
     s.lagged.clear();
-    int rc = build_levels(m, na, omega, levels.data(), s.lagged);
+    int rc = build_levels(m, na, omega, levels.data(), s.lagged, dep_tol);
     if (rc)
         return rc;
     int32_t min_lev = INT32_MAX;
@@ -728,14 +741,14 @@ int t2_mesh_geometry(int64_t n_cells, int64_t n_nodes, const double *coords,
                              volume, area, normal);
 }
 
-int t2_sweep_levels(int64_t n_cells, const int64_t *adj_cell,
-                    const double *area, const double *normal,
-                    int64_t n_angles, const double *omega, int32_t *levels,
-                    int64_t *n_lagged)
+int t2_sweep_levels_tol(int64_t n_cells, const int64_t *adj_cell,
+                        const double *area, const double *normal,
+                        int64_t n_angles, const double *omega,
+                        double dep_tol, int32_t *levels, int64_t *n_lagged)
 {
     t2::clear_error();
     if (n_cells <= 0 || n_angles <= 0 || !adj_cell || !area || !normal ||
-        !omega || !levels)
+        !omega || !levels || std::isnan(dep_tol))
         return t2::fail(T2_ERR_ARG, "t2_sweep_levels: bad argument");
     t2::HostMesh m;
     m.nc = n_cells;
@@ -743,10 +756,20 @@ int t2_sweep_levels(int64_t n_cells, const int64_t *adj_cell,
     m.area.assign(area, area + 4 * n_cells);
     m.normal.assign(normal, normal + 12 * n_cells);
     std::vector<t2::LaggedFace> lagged;
-    int rc = t2::build_levels(m, n_angles, omega, levels, lagged);
+    int rc = t2::build_levels(m, n_angles, omega, levels, lagged,
+                              dep_tol < 0 ? t2::kTraverseRule : dep_tol);
     if (n_lagged)
         *n_lagged = int64_t(lagged.size());
     return rc;
+}
+
+int t2_sweep_levels(int64_t n_cells, const int64_t *adj_cell,
+                    const double *area, const double *normal,
+                    int64_t n_angles, const double *omega, int32_t *levels,
+                    int64_t *n_lagged)
+{
+    return t2_sweep_levels_tol(n_cells, adj_cell, area, normal, n_angles,
+                               omega, t2::kTraverseRule, levels, n_lagged);
 }
 
 

@@ -449,10 +449,13 @@ public:
                             engine == "Queue" ? T2_ENGINE_QUEUE
                                               : T2_ENGINE_LEVELS,
                             0, 0));
-        if (sched == "Priorities")
+        const std::string type = deck.str("SweepType");
+        if (sched == "Priorities" || type != "HIP")
             check(t2_set_priority(c_engine, int(deck.num("intraAngleP")),
                                   int(deck.num("interAngleP")),
                                   int64_t(deck.num("maxCellsPerStep"))));
+        if (type == "OriginalTycho1" || type == "OriginalTycho2")
+            check(t2_set_dependency_tolerance(c_engine, 1e-10));
     }
     ~SweeperHIP() override { t2_sweeper_destroy(c_engine); }
 
@@ -622,10 +625,19 @@ int main(int argc, char **argv)
     }
     try {
         Deck deck(argv[2]);
+        // SweepType (src/Main.cc:143-164): HIP is the engine's level sets;
+        // the reference's single-rank sweepers map onto the engine with
+        // their schedules: TraverseGraph follows the deck's priorities,
+        // OriginalTycho1/2 follow them with the precomputed schedule's
+        // dependency tolerance (src/SweepSchedule.cc:57).  The multi-rank
+        // domain-decomposition sweepers (PBJ, Schur) are not built here.
         const std::string type = deck.str("SweepType");
-        if (type != "HIP")
-            throw std::runtime_error("Sweep type not recognized: " + type +
-                                     " (this executable runs SweepType HIP)");
+        if (type != "HIP" && type != "TraverseGraph" &&
+            type != "OriginalTycho1" && type != "OriginalTycho2")
+            throw std::runtime_error(
+                "Sweep type not recognized: " + type +
+                " (this executable runs SweepType HIP, TraverseGraph, "
+                "OriginalTycho1 and OriginalTycho2)");
         Problem p;
         p.mesh = read_mesh(argv[1]);
         p.sn = int(deck.num("snOrder"));

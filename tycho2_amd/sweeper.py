@@ -61,6 +61,11 @@ SIGNATURES = {
                                      ctypes.c_int]),
     "t2_set_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "t2_set_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "t2_set_dependency_tolerance": (ctypes.c_int, [ctypes.c_void_p,
+                                                   ctypes.c_double]),
+    "t2_sweep_levels_tol": (ctypes.c_int, [ctypes.c_int64, _ip, _dp, _dp,
+                                           ctypes.c_int64, _dp,
+                                           ctypes.c_double, _i32p, _ip]),
     "t2_set_items_per_lane": (ctypes.c_int, [ctypes.c_void_p,
                                              ctypes.c_int]),
     "t2_set_concurrent_batches": (ctypes.c_int, [ctypes.c_void_p,
@@ -238,15 +243,22 @@ def mesh_geometry(smesh, conn):
     return vol, area, nrm
 
 
-def sweep_levels(adj, area, normal, omega):
-    """(levels[a][c], n_lagged) of the engine's per-angle schedule."""
+def sweep_levels(adj, area, normal, omega, dep_tol=None):
+    """(levels[a][c], n_lagged) of the engine's per-angle schedule, with the
+    graph traverser's dependency rule or (dep_tol >= 0) the precomputed
+    schedule's."""
     adj, ar, nrm, om = _i64(adj), _f64(area), _f64(normal), _f64(omega)
     nc, na = adj.shape[0], om.shape[0]
     levels = np.empty((na, nc), dtype=np.int32)
     lag = ctypes.c_int64(0)
-    _check(lib().t2_sweep_levels(nc, _ptr(adj, _ip), _ptr(ar), _ptr(nrm), na,
-                                 _ptr(om), _ptr(levels, _i32p),
-                                 ctypes.byref(lag)))
+    if dep_tol is None:
+        _check(lib().t2_sweep_levels(nc, _ptr(adj, _ip), _ptr(ar), _ptr(nrm),
+                                     na, _ptr(om), _ptr(levels, _i32p),
+                                     ctypes.byref(lag)))
+    else:
+        _check(lib().t2_sweep_levels_tol(
+            nc, _ptr(adj, _ip), _ptr(ar), _ptr(nrm), na, _ptr(om),
+            float(dep_tol), _ptr(levels, _i32p), ctypes.byref(lag)))
     return levels, lag.value
 
 
@@ -348,6 +360,12 @@ class SweeperHIP:
     def set_order(self, order: int) -> None:
         """ORDER_CELL or ORDER_FACES: item order within a step."""
         _check(lib().t2_set_order(self.handle, int(order)))
+
+    def set_dependency_tolerance(self, tol: float) -> None:
+        """Faces with |Omega.n| <= tol are not sweep dependencies and read
+        the pre-sweep psi (SweepType OriginalTycho1/2 use 1e-10); tol < 0
+        returns to the graph traverser's rule."""
+        _check(lib().t2_set_dependency_tolerance(self.handle, float(tol)))
 
     def set_concurrent_batches(self, on: bool) -> None:
         """Level engine: angle batches on parallel streams (True) or one
