@@ -101,6 +101,21 @@ def path_leg(p, precision, steps, warmup, ev, tune, solve, launch="",
         out["ms_per_sweep"] = time_region(ev, s, s.sweep_only, steps) / steps
         log(f"precision {precision}: {out['ms_per_step']:.2f} ms/step, "
             f"{out['ms_per_sweep']:.2f} ms/sweep")
+        # The sweep kernel's own launch duration: the same schedule and
+        # launches replayed on one stream (concurrent angle batches off), so
+        # that the sweep's time over its launches is the kernel's average
+        # duration -- what a kernel trace of the run reports (the tracer
+        # serialises the streams of the concurrent form).
+        out["ms_per_sweep_serial"] = out["ms_per_sweep"]
+        if out["streams"] > 1:
+            s.set_concurrent_batches(False)
+            s.sweep_only()
+            s.synchronize()
+            out["ms_per_sweep_serial"] = time_region(ev, s, s.sweep_only,
+                                                     steps) / steps
+            s.set_concurrent_batches(True)
+            log(f"precision {precision}: {out['ms_per_sweep_serial']:.2f} "
+                f"ms/sweep with the batches' launches on one stream")
         out["n_steps"] = info["n_steps"]
         out["psi_gb"] = info["psi_bytes"] / 1e9
         if solve:
@@ -146,6 +161,29 @@ def path_leg(p, precision, steps, warmup, ev, tune, solve, launch="",
     finally:
         s.close()
     return out
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores() -> tuple[int, int]:
+    """(threads to use, logical CPUs in this process's affinity): the
+    affinity, bounded by OMP_NUM_THREADS where the environment sets it (the
+    host's CPU share for this process)."""
+    logical = len(os.sched_getaffinity(0))
+    n = logical
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n, logical
 
 
 def cpu_leg(p, n_angles, threads, reps=5):
@@ -574,8 +612,8 @@ def main() -> int:
                     help="comma-separated waves of the batch workload to run "
                          "(default: all)")
     ap.add_argument("--cpu-angles", type=int, default=0,
-                    help="angles in the CPU-baseline sample (0: one per "
-                         "core)")
+                    help="angles in the CPU-baseline sample (at least one "
+                         "per core)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the two solves to 1e-10 (steps only)")
@@ -662,7 +700,14 @@ def main() -> int:
             launches = r["n_steps"]
         # concurrent angle batches: `streams` chains of launches
         streams = r.get("streams", 1) if r.get("engine", 0) == 0 else 1
-        ach = solves * bytes_per_solve / (r["ms_per_sweep"] * 1e-3) / 1e9
+        per_launch = solves * bytes_per_solve / launches
+        # the kernel's average launch duration: the schedule's launches on
+        # one stream (= the sweep's time / launches, no gaps in a replay)
+        avg_ms = r["ms_per_sweep_serial"] / launches
+        ach = per_launch / (avg_ms * 1e-3) / 1e9
+        # the tuned concurrent form: launches overlap, so its time per launch
+        # is an interval, not a kernel duration
+        ach_c = solves * bytes_per_solve / (r["ms_per_sweep"] * 1e-3) / 1e9
         return {"bound": "hbm", "kernel": kernel, "achieved": round(ach, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4),
@@ -670,14 +715,14 @@ def main() -> int:
                                                  f"fp{8 * width}",
                                                  launches),
                 "alg_bytes_per_solve": round(bytes_per_solve, 3),
-                "alg_bytes_per_launch": round(solves * bytes_per_solve /
-                                              launches),
-                # the sweep's time per launch: with `streams` concurrent
-                # chains the launches overlap, so this is the interval, not
-                # one kernel's duration
-                "avg_launch_ms": round(r["ms_per_sweep"] / launches, 5),
+                "alg_bytes_per_launch": round(per_launch),
+                "avg_launch_ms": round(avg_ms, 5),
                 "launches_per_sweep": launches,
-                "streams": streams}
+                "streams": streams,
+                "concurrent_interval_ms": round(r["ms_per_sweep"] / launches,
+                                                5),
+                "achieved_concurrent": round(ach_c, 1),
+                "frac_concurrent": round(ach_c / HBM_PEAK_GBS, 4)}
 
     plain = {"solves_per_s_step": round(solves / (r64["ms_per_step"] * 1e-3)),
              "solves_per_s_sweep": round(solves /
@@ -748,16 +793,19 @@ def main() -> int:
            "roofline": roofline, "plain_fp64": plain,
            "refined_fp32_fp64": refined}
     if not args.no_cpu:
-        cores = min(16, len(os.sched_getaffinity(0)))
-        na_cpu = args.cpu_angles or min(p.na, cores)
+        cores, logical = host_cores()
+        # at least one angle per thread (the oracle's threads take angles)
+        na_cpu = min(p.na, max(args.cpu_angles, cores))
         rate, dt = cpu_leg(p, na_cpu, cores)
         out["cpu_baseline"] = {
             "value": round(rate, 1), "unit": "solves/s", "cores": cores,
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(),
+            "logical_cpus_in_affinity": logical,
             "sample": f"median of 5 oracle FP64 sweeps of angles "
                       f"0..{na_cpu - 1} of the same mesh, {p.ng} groups, "
-                      f"{cores} OpenMP threads (one angle each), {dt:.1f}s "
-                      f"in total"}
+                      f"{cores} OpenMP threads (every core of this "
+                      f"process's CPU share; at least one angle each), "
+                      f"{dt:.1f}s in total"}
     print(json.dumps(out), flush=True)
     return 0
 
