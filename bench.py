@@ -204,6 +204,26 @@ def cpu_leg(p, n_angles, threads, reps=5):
     return p.nc * n_angles * p.ng / float(np.median(times)), sum(times)
 
 
+def solve_run(p, precision, launch="", batches=1, concurrent=False):
+    """A short program for a kernel trace of one path's solve: one solve
+    (FP64 GMRES(20) to the problem's tolerance, or the refined solve) after
+    one untimed solve that makes the device allocations."""
+    from tycho2_amd.sweeper import FP32, SweeperHIP
+    cfg = p.solver
+    with SweeperHIP.from_problem(p, precision) as s:
+        fixed_launch(s, launch, batches, concurrent)
+        for rep in range(2):
+            t0 = time.time()
+            if precision == FP32:
+                info, _ = s.solve_refined(cfg["iter_max"], 1e-6, 1e-5,
+                                          cfg["restart"])
+            else:
+                info = len(s.solve_krylov(cfg["iter_max"], cfg["tol"],
+                                          cfg["restart"]))
+            s.synchronize()
+            log(f"solve run {rep}: {time.time() - t0:.3f} s, {info}")
+
+
 def counter_run(p, precision, sweeps, launch="", batches=1,
                 concurrent=False):
     """A short program for rocprofv3 --pmc: `sweeps` sweeps, then a 4 GiB
@@ -636,6 +656,9 @@ def main() -> int:
                     help="maxCellsPerStep values of the schedule study")
     ap.add_argument("--counter-run", choices=["fp64", "fp32"],
                     help="only a short run for rocprofv3 --pmc")
+    ap.add_argument("--solve-run", choices=["fp64", "fp32"],
+                    help="only two solves of one path (for a kernel trace; "
+                         "the fixed launch flags apply)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only (asked for --gpus "
@@ -662,6 +685,10 @@ def main() -> int:
         counter_run(p, FP64 if args.counter_run == "fp64" else FP32,
                     max(1, args.steps), args.launch, args.batches,
                     args.concurrent)
+        return 0
+    if args.solve_run:
+        solve_run(p, FP64 if args.solve_run == "fp64" else FP32, args.launch,
+                  args.batches, args.concurrent)
         return 0
     ev = HipEvents()
     if args.schedule_study:

@@ -279,6 +279,26 @@ int t2_set_source_angular(t2_sweeper *s, const double *source);
 int t2_solve_source_iteration(t2_sweeper *s, int64_t iter_max, double err_max,
                               int64_t *n_iter, double *err_hist);
 
+/* t2_solve_source_iteration without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * once their handle exists (serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, arithmetic and results, bit for bit; the
+ * L1-change reduction keeps its partial sums in a fixed device array of the
+ * library (2 x 1024 + 2 doubles, 16 KiB, one copy per GPU, held until the
+ * process ends) instead of the handle's buffers, and the call makes no
+ * allocation, graph instantiation or stream creation: a sweep whose hipGraph
+ * is not captured yet is launched step by step.  The array's size does not
+ * depend on the problem (the reduction runs 1024 blocks over any length),
+ * so no problem size exceeds it.  Returns T2_ERR_STATE when a resource
+ * would have to be created first: the ready-queue tables of
+ * T2_ENGINE_QUEUE / T2_ENGINE_HYBRID and the streams of concurrent angle
+ * batches are made by the handle's first ordinary sweep or solve.  Every
+ * call on one GPU uses that GPU's one array: calls that overlap in time on
+ * the same GPU (from several host threads) race and must not be made. */
+int t2_solve_source_iteration_noalloc(t2_sweeper *s, int64_t iter_max,
+                                      double err_max, int64_t *n_iter,
+                                      double *err_hist);
+
 /* GMRES(restart) on (I - D L^-1 M S) phi = D L^-1 q with x0 = 0, then one
  * sweep for psi (src/SourceIteration.cc:216-285).  Arnoldi, Gram-Schmidt and
  * the iterate are FP64 on the device whatever the sweep precision.

@@ -7,12 +7,21 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
 
 #include "t2_common.hpp"
 #include "t2_kernels.hpp"
+
+// Device scratch of t2_solve_source_iteration_noalloc: the L1-change
+// reduction's per-block partial sums (2 x kRedBlocks) and its two totals.
+// Every GPU the library runs on holds one copy until the process ends.  Host
+// code uses only the per-device address that hipGetSymbolAddress returns
+// (noalloc_scratch below), never the array's name.
+constexpr int kNoAllocScratch = 2 * t2::kRedBlocks + 2;
+static __device__ __align__(256) double scratch[kNoAllocScratch];
 
 namespace {
 
@@ -56,6 +65,7 @@ struct t2_sweeper {
     int64_t step_width = 0;
     double dep_tol = t2::kTraverseRule;   // dependency rule of the schedule
     bool use_graph = true;
+    bool no_alloc = false;           // inside an allocation-free entry point
     bool concurrent = false;         // angle batches on parallel streams
     std::vector<hipStream_t> side;   // their extra streams
     std::vector<hipEvent_t> join_ev;
@@ -158,6 +168,29 @@ int dalloc_t(t2_sweeper *s, P **p, size_t bytes)
     int rc = dalloc(s, &v, bytes);
     *p = static_cast<P *>(v);
     return rc;
+}
+
+// Address of this device's copy of `scratch`, looked up once per device
+// (the device must be current).  A fixed table: the lookup allocates nothing.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+int noalloc_scratch(int device, double **out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices)
+        return fail(T2_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        T2_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p)
+            return fail(T2_ERR_HIP, "hipGetSymbolAddress(scratch): null");
+        g_scratch_dev[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch_dev[device];
+    return T2_OK;
 }
 
 template <typename P>
@@ -536,13 +569,32 @@ int run_sweep(t2_sweeper *s, const double *ang, const void *ang_t = nullptr,
         psi_out = s->d_psi;
     if (psi_out == s->d_psi)   // psi is replaced: no correction applies
         s->psi_corrected = false;
+    const int64_t n_side = int64_t(s->sched.batch_steps.size()) - 2;
+    if (s->no_alloc) {
+        // nothing may be created here: the ready-queue tables and the
+        // streams of concurrent batches must exist already
+        int64_t first = 0;
+        if (s->engine != T2_ENGINE_LEVELS) {
+            T2_TRY(queue_first_step(s, &first));
+            if (s->n_chunks < 0 || s->queue_step != first)
+                return fail(T2_ERR_STATE,
+                            "allocation-free call: the ready-queue tables "
+                            "of this engine and schedule are not built "
+                            "(an ordinary sweep or solve builds them)");
+        }
+        if (s->concurrent && n_side > 0 &&
+            (!s->fork_ev || int64_t(s->side.size()) < n_side))
+            return fail(T2_ERR_STATE,
+                        "allocation-free call: the streams of concurrent "
+                        "angle batches are not created (an ordinary sweep "
+                        "or solve creates them)");
+    }
     T2_TRY(ensure_queue(s));
     T2_TRY(check_step_launches(s->engine == T2_ENGINE_LEVELS
                                    ? s->sched.n_steps
                                    : s->queue_step));
-    if (s->concurrent)   // created outside any stream capture
-        T2_TRY(ensure_side_streams(
-            s, int64_t(s->sched.batch_steps.size()) - 2));
+    if (s->concurrent && !s->no_alloc)   // created outside any capture
+        T2_TRY(ensure_side_streams(s, n_side));
     s->n_sweeps++;
     if (!s->use_graph)
         return enqueue_sweep_any(s, ang, ang_t, psi_out);
@@ -550,6 +602,8 @@ int run_sweep(t2_sweeper *s, const double *ang, const void *ang_t = nullptr,
     for (auto &g : s->graphs)
         if (g.ang == ang && g.ang_t == ang_t && g.out == psi_out)
             exec = g.exec;
+    if (!exec && s->no_alloc)   // no capture: the same launches, direct
+        return enqueue_sweep_any(s, ang, ang_t, psi_out);
     if (!exec) {
         hipGraph_t g = nullptr;
         T2_HIP(hipStreamBeginCapture(s->stream,
@@ -1227,7 +1281,10 @@ int t2_sweeper_create(t2_sweeper **out, int device, int64_t n_cells,
         delete s;
         return fail(T2_ERR_HIP, "stream creation failed");
     }
-    rc = setup_device(s, weights, sigma_s);
+    double *scr = nullptr;   // this device's scratch address, once
+    rc = noalloc_scratch(device, &scr);
+    if (rc == T2_OK)
+        rc = setup_device(s, weights, sigma_s);
     if (rc != T2_OK) {
         std::string msg = t2_last_error();
         t2_sweeper_destroy(s);
@@ -1528,8 +1585,11 @@ int t2_sweep_only(t2_sweeper *s)
     return run_sweep(s, s->d_qang);
 }
 
+// partial / sums: the L1-change reduction's 2 x kRedBlocks partial sums and
+// its two totals (the handle's, or the allocation-free entry point's scratch)
 static int solve_si_body(t2_sweeper *s, int64_t iter_max, double err_max,
-                              int64_t *n_iter, double *err_hist)
+                         int64_t *n_iter, double *err_hist, double *partial,
+                         double *sums_dev)
 {
     t2::clear_error();
     T2_TRY(check(s));
@@ -1542,9 +1602,9 @@ static int solve_si_body(t2_sweeper *s, int64_t iter_max, double err_max,
     while (it < iter_max && err > err_max) {
         T2_TRY(t2_sweep_step(s));
         T2_HIP(t2::launch_l1_change(s->d_phi, s->d_phi_old, s->n_phi(),
-                                    s->d_red, s->d_scal, s->stream));
+                                    partial, sums_dev, s->stream));
         double sums[2];
-        T2_TRY(fetch(s, sums, s->d_scal, 2));
+        T2_TRY(fetch(s, sums, sums_dev, 2));
         err = sums[0] / sums[1];
         if (err_hist)
             err_hist[it] = err;
@@ -1943,8 +2003,28 @@ int t2_sweep_host(t2_sweeper *s, double *psi, const double *source,
 int t2_solve_source_iteration(t2_sweeper *s, int64_t iter_max, double err_max,
                               int64_t *n_iter, double *err_hist)
 {
+    if (!s)
+        return solve_si_body(s, iter_max, err_max, n_iter, err_hist, nullptr,
+                             nullptr);
     return after_queue(s, solve_si_body(s, iter_max, err_max, n_iter,
-                                        err_hist));
+                                        err_hist, s->d_red, s->d_scal));
+}
+
+int t2_solve_source_iteration_noalloc(t2_sweeper *s, int64_t iter_max,
+                                      double err_max, int64_t *n_iter,
+                                      double *err_hist)
+{
+    t2::clear_error();
+    T2_TRY(check(s));
+    double *scr = nullptr;
+    T2_TRY(noalloc_scratch(s->device, &scr));
+    // the partial sums and totals are written in full by every reduction
+    // before they are read: nothing in the scratch needs clearing
+    s->no_alloc = true;
+    const int rc = solve_si_body(s, iter_max, err_max, n_iter, err_hist, scr,
+                                 scr + 2 * t2::kRedBlocks);
+    s->no_alloc = false;
+    return after_queue(s, rc);
 }
 
 int t2_solve_krylov(t2_sweeper *s, int64_t iter_max, double rtol,

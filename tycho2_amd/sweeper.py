@@ -90,6 +90,8 @@ SIGNATURES = {
     "t2_set_source_angular": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "t2_solve_source_iteration": (ctypes.c_int, [
         ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, _ip, _dp]),
+    "t2_solve_source_iteration_noalloc": (ctypes.c_int, [
+        ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, _ip, _dp]),
     "t2_solve_krylov": (ctypes.c_int, [
         ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int64,
         _ip, _dp]),
@@ -420,12 +422,16 @@ class SweeperHIP:
         src = _f64(source).reshape(self.nc, self.na, 4, self.ng)
         _check(lib().t2_set_source_angular(self.handle, _ptr(src)))
 
-    def solve_source_iteration(self, iter_max=100, err_max=1e-10):
+    def solve_source_iteration(self, iter_max=100, err_max=1e-10,
+                               no_alloc=False):
+        """Source iteration on the device; no_alloc=True calls the
+        allocation-free entry point (same results)."""
         hist = np.zeros(max(int(iter_max), 1))
         nit = ctypes.c_int64(0)
-        rc = lib().t2_solve_source_iteration(
-            self.handle, int(iter_max), float(err_max), ctypes.byref(nit),
-            _ptr(hist))
+        fn = (lib().t2_solve_source_iteration_noalloc if no_alloc
+              else lib().t2_solve_source_iteration)
+        rc = fn(self.handle, int(iter_max), float(err_max), ctypes.byref(nit),
+                _ptr(hist))
         return _solved(rc, hist[:nit.value].copy())
 
     def solve_krylov(self, iter_max=100, rtol=1e-10, restart=30):
